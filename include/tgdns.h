@@ -1,0 +1,128 @@
+/* tgdns.h - C ABI of the MI355X pseudo-spectral Taylor-Green DNS library
+ * (libtgdns.so).
+ *
+ * State layout ("compact spectrum"): for a grid of N^3 points, M = N/3,
+ * K = 2M+1, KZ = M+1, each vector component is a C-order array
+ * [K][K][KZ] of complex numbers (interleaved re,im; double for precision 64,
+ * float for 32) holding exactly the Fourier modes the reference keeps after
+ * its inclusive 2/3 cube mask (|k_i| <= N/3), kz >= 0 only (the field is
+ * real, so kz < 0 follows by Hermitian symmetry).  Compact index i maps to
+ * the signed wavenumber i (i <= M) or i-K (i > M), times 2*pi/L.  A vector
+ * field is 3 such components back to back.  Transforms follow numpy/JAX
+ * conventions: forward unnormalised, inverse divided by N^3.
+ *
+ * Physical fields are C-order [3][N][N][N] real arrays (component, x, y, z),
+ * the reference's own layout.
+ *
+ * Conventions: every entry point returns 0 on success and a nonzero TG_ERR_*
+ * code otherwise; tg_last_error() then returns the message (thread-local).
+ * Every pointer argument except `out`/`info` host arrays is a DEVICE pointer
+ * owned by the caller; the library owns only the plan (twiddle tables).
+ * `stream` is a hipStream_t (NULL = default stream); work is enqueued on it
+ * and the call returns without synchronising, except tg_diagnostics, which
+ * copies its scalars to the host and synchronises the stream.
+ * A plan is not thread-safe: use one plan per host thread / stream.
+ * `ws` is a device workspace of at least tg_plan_query(...)[TG_INFO_WS_BYTES]
+ * bytes; its contents need not persist between calls.
+ */
+#ifndef TGDNS_H
+#define TGDNS_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TG_OK 0
+#define TG_ERR_ARG 1
+#define TG_ERR_HIP 2
+#define TG_ERR_UNSUPPORTED 3
+
+#define TG_INFO_N 0
+#define TG_INFO_K 1
+#define TG_INFO_KZ 2
+#define TG_INFO_COMPACT_ELEMS 3 /* complex elements per component: K*K*KZ */
+#define TG_INFO_WS_BYTES 4
+#define TG_INFO_PRECISION 5
+#define TG_INFO_MHD 6
+#define TG_INFO_COUNT 8
+
+typedef struct tg_plan tg_plan;
+
+/* Library ABI version (monotone integer). */
+int tg_version(void);
+
+/* Grid sizes with compiled plans, written to out[0..cap); returns the count. */
+int tg_supported_sizes(int* out, int cap);
+
+/* Create a plan for an N^3 grid of side `domain_length` (2*pi in every
+ * reference configuration).  precision_bits: 64 or 32.  mhd: 0 = hydro
+ * (velocity only), 1 = MHD (velocity + magnetic field).
+ * Replaces the reference's per-call setup: make_wavenumbers /
+ * make_dealias_mask (jax/taylor_green_jax.py:55-67) and
+ * SpectralGrid.__init__ (cuda/taylor_green_cuda/grid.py:5-13). */
+int tg_plan_create(int n, int precision_bits, int mhd, double domain_length, tg_plan** out);
+int tg_plan_destroy(tg_plan* plan);
+
+/* info[TG_INFO_COUNT] receives the TG_INFO_* values. */
+int tg_plan_query(const tg_plan* plan, long long* info);
+
+/* Physical [3][N][N][N] real -> compact spectrum (3 components), i.e.
+ * mask * fftn(u) restricted to the kept modes.
+ * Replaces fftn(u) * dealias_mask (jax/taylor_green_jax.py:160-161) and
+ * SpectralGrid.to_spectral (cuda/taylor_green_cuda/grid.py:31-34). */
+int tg_forward(tg_plan* plan, const void* phys, void* spec, void* ws, void* stream);
+
+#define TG_OP_FIELD 0 /* the field itself          -> phys [3][N][N][N] */
+#define TG_OP_CURL 1  /* i k x field (vorticity)   -> phys [3][N][N][N] */
+#define TG_OP_DIV 2   /* i k . field (divergence)  -> phys [N][N][N]    */
+
+/* Compact spectrum (3 components) -> physical real field(s): ifftn(...).real
+ * of the Hermitian full-cube spectrum of the field, its curl or divergence.
+ * Replaces ifftn(...).real (jax/taylor_green_jax.py:185),
+ * SpectralGrid.to_real (cuda/taylor_green_cuda/grid.py:36-37), vorticity
+ * (jax/taylor_green_jax.py:99-109; MHD spectral_curl operators.py:24-25) and
+ * spectral_divergence (jax/taylor_green_jax.py:90-96; MHD
+ * divergence_from_hat operators.py:50-51). */
+int tg_inverse(tg_plan* plan, const void* spec, int op, void* phys, void* ws, void* stream);
+
+/* Right-hand side in the compact spectrum.
+ * hydro: r_u = P(u x omega) - nu k^2 u   (== spectral_rhs,
+ *        jax/taylor_green_jax.py:158-185, for band-limited u);
+ * MHD:   r_u = P(u x omega + lorentz_sign * j x b) - nu k^2 u,
+ *        r_b = P(i k x (u x b)) - eta k^2 b
+ *        (== MhdSpectralOperator.rhs, cuda/taylor_green_mhd/operators.py:70-107,
+ *        with lorentz_sign = -1; +1 is cuda/README_MHD.md:9).
+ * b_hat / rb_hat are ignored (may be NULL) for hydro plans. */
+int tg_rhs(tg_plan* plan, const void* u_hat, const void* b_hat, double nu, double eta,
+           double lorentz_sign, void* ru_hat, void* rb_hat, void* ws, void* stream);
+
+/* nsteps classic RK4 steps in place on the compact state.
+ * Replaces advance_one_step (jax/taylor_green_jax.py:188-196),
+ * MhdRK4Integrator.step (cuda/taylor_green_mhd/integrator.py:14-32) and the
+ * MHD shim advance_one_step (cuda/taylor_green_mhd.py:49-59). */
+int tg_rk4_steps(tg_plan* plan, void* u_hat, void* b_hat, int nsteps, double dt, double nu,
+                 double eta, double lorentz_sign, void* ws, void* stream);
+
+/* Diagnostics of the compact state, FP64 compensated sums (also for FP32
+ * plans).  out[0..9]:
+ *  0 E_k = 1/2 <|u|^2>                 (kinetic_energy, jax/...:199-201)
+ *  1 Z   = 1/2 <|curl u|^2>            (enstrophy, jax/...:112-115)
+ *  2 <|grad u|^2>  (x nu = viscous_dissipation, jax/...:118-128)
+ *  3 E_m = 1/2 <|b|^2>                 (cuda/taylor_green_mhd/diagnostics.py:23)
+ *  4 H_c = <u . b>                     (diagnostics.py:25-29)
+ *  5 <|div u|^2>   6 <|div b|^2>   7 <|curl b|^2>
+ *  8 max |div u| on the grid, 9 max |div b| (only if with_divmax, else 0)
+ * Synchronises `stream`. */
+int tg_diagnostics(tg_plan* plan, const void* u_hat, const void* b_hat, int with_divmax,
+                   double* out, void* ws, void* stream);
+
+/* Copy the calling thread's last error message into buf (NUL-terminated). */
+int tg_last_error(char* buf, size_t n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TGDNS_H */

@@ -1,0 +1,247 @@
+// Stockham FFT engine for gfx950: batches of lines held in LDS, one radix-R
+// butterfly group per thread per stage, twiddles from a host-built table.
+//
+// Layout convention: element n of line l lives at buf[n * LP + l].  A thread
+// owns line l = tid % L and slice t = tid / L of that line; in every stage it
+// loads E/R butterflies of R points at stride N/R, applies twiddles, does the
+// radix-R DFT in registers and writes the results at stride P (the product of
+// the radices already applied).  After the last stage the line is in natural
+// order (autosort), so no bit-reversal pass exists.
+//
+// DIR = -1 is the forward transform  X[k] = sum_n x[n] exp(-2 pi i n k / N),
+// DIR = +1 the unnormalised inverse  x[n] = sum_k X[k] exp(+2 pi i n k / N).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace tg {
+
+template <typename R>
+struct alignas(2 * sizeof(R)) cpx {
+  R x, y;
+};
+
+template <typename R>
+__device__ __forceinline__ cpx<R> cadd(cpx<R> a, cpx<R> b) { return {a.x + b.x, a.y + b.y}; }
+template <typename R>
+__device__ __forceinline__ cpx<R> csub(cpx<R> a, cpx<R> b) { return {a.x - b.x, a.y - b.y}; }
+template <typename R>
+__device__ __forceinline__ cpx<R> cmul(cpx<R> a, cpx<R> b) {
+  return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <typename R>
+__device__ __forceinline__ cpx<R> cscale(cpx<R> a, R s) { return {a.x * s, a.y * s}; }
+// multiply by i
+template <typename R>
+__device__ __forceinline__ cpx<R> cmuli(cpx<R> a) { return {-a.y, a.x}; }
+
+// multiply by exp(DIR * 2 pi i / 4) = DIR * i
+template <int DIR, typename R>
+__device__ __forceinline__ cpx<R> rot4(cpx<R> a) {
+  if constexpr (DIR < 0) return {a.y, -a.x};
+  else return {-a.y, a.x};
+}
+
+// ---------------------------------------------------------------- radix DFTs
+template <int DIR, typename R>
+__device__ __forceinline__ void dft2(cpx<R>* v) {
+  cpx<R> a = v[0];
+  v[0] = cadd(a, v[1]);
+  v[1] = csub(a, v[1]);
+}
+
+template <int DIR, typename R>
+__device__ __forceinline__ void dft3(cpx<R>* v) {
+  // exp(DIR 2 pi i / 3) = -1/2 + DIR i sqrt(3)/2
+  const R h = R(0.86602540378443864676372317075294);  // sqrt(3)/2
+  cpx<R> s = cadd(v[1], v[2]);
+  cpx<R> d = csub(v[1], v[2]);
+  cpx<R> m = {v[0].x - R(0.5) * s.x, v[0].y - R(0.5) * s.y};
+  cpx<R> r = {-R(DIR) * h * d.y, R(DIR) * h * d.x};  // DIR i h d
+  v[0] = cadd(v[0], s);
+  v[1] = cadd(m, r);
+  v[2] = csub(m, r);
+}
+
+template <int DIR, typename R>
+__device__ __forceinline__ void dft4s(cpx<R>& x0, cpx<R>& x1, cpx<R>& x2, cpx<R>& x3) {
+  cpx<R> t0 = cadd(x0, x2), t1 = csub(x0, x2);
+  cpx<R> t2 = cadd(x1, x3), t3 = rot4<DIR>(csub(x1, x3));
+  x0 = cadd(t0, t2);
+  x2 = csub(t0, t2);
+  x1 = cadd(t1, t3);
+  x3 = csub(t1, t3);
+}
+
+template <int DIR, typename R>
+__device__ __forceinline__ void dft4(cpx<R>* v) { dft4s<DIR>(v[0], v[1], v[2], v[3]); }
+
+template <int DIR, typename R>
+__device__ __forceinline__ cpx<R> mul_w8(cpx<R> a) {  // * exp(DIR 2 pi i/8)
+  const R c = R(0.70710678118654752440084436210485);
+  return {c * (a.x - R(DIR) * a.y), c * (a.y + R(DIR) * a.x)};
+}
+template <int DIR, typename R>
+__device__ __forceinline__ cpx<R> mul_w8_3(cpx<R> a) {  // * exp(DIR 3 * 2 pi i/8)
+  const R c = R(0.70710678118654752440084436210485);
+  return {c * (-a.x - R(DIR) * a.y), c * (R(DIR) * a.x - a.y)};
+}
+
+template <int DIR, typename R>
+__device__ __forceinline__ void dft8(cpx<R>* v) {
+  cpx<R> e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+  cpx<R> o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+  dft4s<DIR>(e0, e1, e2, e3);
+  dft4s<DIR>(o0, o1, o2, o3);
+  o1 = mul_w8<DIR>(o1);
+  o2 = rot4<DIR>(o2);
+  o3 = mul_w8_3<DIR>(o3);
+  v[0] = cadd(e0, o0); v[4] = csub(e0, o0);
+  v[1] = cadd(e1, o1); v[5] = csub(e1, o1);
+  v[2] = cadd(e2, o2); v[6] = csub(e2, o2);
+  v[3] = cadd(e3, o3); v[7] = csub(e3, o3);
+}
+
+template <int DIR, typename R>
+__device__ __forceinline__ void dft16(cpx<R>* v) {
+  // n = 4 n1 + n2, k = k1 + 4 k2: inner DFT4 over n1, twiddle w16^(n2 k1),
+  // outer DFT4 over n2.
+  const R c1 = R(0.92387953251128675612818318939679);  // cos(pi/8)
+  const R s1 = R(0.38268343236508977172845998403040);  // sin(pi/8)
+  for (int n2 = 0; n2 < 4; ++n2) dft4s<DIR>(v[n2], v[4 + n2], v[8 + n2], v[12 + n2]);
+  // now v[4*k1 + n2] holds A[n2][k1]
+  // w16^1 = (c1, DIR s1), w16^2 = w8, w16^3 = (s1, DIR c1), w16^4 = DIR i,
+  // w16^6 = w8^3, w16^9 = (-c1, -DIR s1)
+  auto w1 = [&](cpx<R> a) -> cpx<R> {
+    return {a.x * c1 - a.y * R(DIR) * s1, a.x * R(DIR) * s1 + a.y * c1};
+  };
+  auto w3 = [&](cpx<R> a) -> cpx<R> {
+    return {a.x * s1 - a.y * R(DIR) * c1, a.x * R(DIR) * c1 + a.y * s1};
+  };
+  auto w9 = [&](cpx<R> a) -> cpx<R> {
+    return {-a.x * c1 + a.y * R(DIR) * s1, -a.x * R(DIR) * s1 - a.y * c1};
+  };
+  // k1 = 1: n2 = 1,2,3 -> w^1, w^2, w^3
+  v[5] = w1(v[5]); v[6] = mul_w8<DIR>(v[6]); v[7] = w3(v[7]);
+  // k1 = 2: w^2, w^4, w^6
+  v[9] = mul_w8<DIR>(v[9]); v[10] = rot4<DIR>(v[10]); v[11] = mul_w8_3<DIR>(v[11]);
+  // k1 = 3: w^3, w^6, w^9
+  v[13] = w3(v[13]); v[14] = mul_w8_3<DIR>(v[14]); v[15] = w9(v[15]);
+  cpx<R> o[16];
+  for (int k1 = 0; k1 < 4; ++k1) {
+    cpx<R> a0 = v[4 * k1 + 0], a1 = v[4 * k1 + 1], a2 = v[4 * k1 + 2], a3 = v[4 * k1 + 3];
+    dft4s<DIR>(a0, a1, a2, a3);
+    o[k1 + 0] = a0; o[k1 + 4] = a1; o[k1 + 8] = a2; o[k1 + 12] = a3;
+  }
+  for (int i = 0; i < 16; ++i) v[i] = o[i];
+}
+
+template <int RAD, int DIR, typename R>
+__device__ __forceinline__ void dft(cpx<R>* v) {
+  if constexpr (RAD == 2) dft2<DIR>(v);
+  else if constexpr (RAD == 3) dft3<DIR>(v);
+  else if constexpr (RAD == 4) dft4<DIR>(v);
+  else if constexpr (RAD == 8) dft8<DIR>(v);
+  else if constexpr (RAD == 16) dft16<DIR>(v);
+  else static_assert(RAD == 2, "unsupported radix");
+}
+
+// ---------------------------------------------------------------- plans
+// Plan<N, E>::run applies the stages for a transform of length N where every
+// thread holds E points; each radix divides E and the radices multiply to N.
+template <int... Rs>
+struct Radices {};
+
+template <int N, int E>
+struct PlanFor;
+#define TG_PLAN(N_, E_, ...) \
+  template <> struct PlanFor<N_, E_> { using type = Radices<__VA_ARGS__>; };
+TG_PLAN(4, 2, 2, 2)
+TG_PLAN(4, 4, 4)
+TG_PLAN(8, 4, 4, 2)
+TG_PLAN(8, 8, 8)
+TG_PLAN(16, 4, 4, 4)
+TG_PLAN(16, 8, 8, 2)
+TG_PLAN(16, 16, 16)
+TG_PLAN(32, 4, 4, 4, 2)
+TG_PLAN(32, 8, 8, 4)
+TG_PLAN(32, 16, 16, 2)
+TG_PLAN(48, 12, 4, 4, 3)
+TG_PLAN(48, 24, 8, 3, 2)
+TG_PLAN(64, 4, 4, 4, 4)
+TG_PLAN(64, 8, 8, 8)
+TG_PLAN(64, 16, 16, 4)
+TG_PLAN(96, 12, 4, 4, 2, 3)
+TG_PLAN(96, 24, 8, 4, 3)
+TG_PLAN(128, 8, 8, 8, 2)
+TG_PLAN(128, 16, 16, 8)
+TG_PLAN(256, 8, 8, 8, 4)
+TG_PLAN(256, 16, 16, 16)
+#undef TG_PLAN
+
+// One Stockham stage of radix RAD after P points have been combined.
+template <typename R, int N, int E, int DIR, int P, int RAD>
+__device__ __forceinline__ void stockham_stage(cpx<R>* buf, int LP, int l, int t, bool active,
+                                               const cpx<R>* __restrict__ tw) {
+  static_assert(E % RAD == 0, "radix must divide points per thread");
+  constexpr int NB = E / RAD;
+  constexpr int SIN = N / RAD;
+  cpx<R> v[NB][RAD];
+  if (active) {
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+      const int b = t * NB + j;
+#pragma unroll
+      for (int m = 0; m < RAD; ++m) v[j][m] = buf[(b + m * SIN) * LP + l];
+    }
+  }
+  __syncthreads();
+  if (active) {
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+      const int b = t * NB + j;
+      const int k = b % P;
+      if constexpr (P > 1) {
+#pragma unroll
+        for (int m = 1; m < RAD; ++m) {
+          cpx<R> w = tw[m * k * (N / (P * RAD))];
+          if constexpr (DIR > 0) w.y = -w.y;
+          v[j][m] = cmul(v[j][m], w);
+        }
+      }
+      dft<RAD, DIR>(v[j]);
+      const int base = (b - k) * RAD + k;
+#pragma unroll
+      for (int m = 0; m < RAD; ++m) buf[(base + m * P) * LP + l] = v[j][m];
+    }
+  }
+  __syncthreads();
+}
+
+template <typename R, int N, int E, int DIR, int P, int R0, int... Rest>
+__device__ __forceinline__ void run_stages(cpx<R>* buf, int LP, int l, int t, bool active,
+                                           const cpx<R>* __restrict__ tw) {
+  stockham_stage<R, N, E, DIR, P, R0>(buf, LP, l, t, active, tw);
+  if constexpr (sizeof...(Rest) > 0)
+    run_stages<R, N, E, DIR, P * R0, Rest...>(buf, LP, l, t, active, tw);
+}
+
+template <typename R, int N, int E, int DIR, int... Rs>
+__device__ __forceinline__ void fft_apply(cpx<R>* buf, int LP, int l, int t, bool active,
+                                          const cpx<R>* __restrict__ tw, Radices<Rs...>) {
+  run_stages<R, N, E, DIR, 1, Rs...>(buf, LP, l, t, active, tw);
+}
+
+// Transform `nlines` lines of length N held in LDS (buf[n*LP + l]).  The
+// calling block must have at least nlines * N / E threads; the rest idle but
+// take part in the barriers.  Entry and exit are barrier-separated.
+template <typename R, int N, int E, int DIR>
+__device__ __forceinline__ void fft_lines(cpx<R>* buf, int LP, int nlines,
+                                          const cpx<R>* __restrict__ tw) {
+  const int tid = threadIdx.x;
+  const int l = tid % nlines;
+  const int t = tid / nlines;
+  const bool active = t < N / E;
+  fft_apply<R, N, E, DIR>(buf, LP, l, t, active, tw, typename PlanFor<N, E>::type{});
+}
+
+}  // namespace tg

@@ -1,0 +1,586 @@
+// The five fused passes of one pseudo-spectral right-hand side on the
+// pruned (2/3-truncated) half spectrum, plus the diagnostics reductions.
+//
+// Index conventions (all arrays C order, z fastest):
+//   N    grid points per axis,  M = N/3 (integer), K = 2M+1, KZ = M+1
+//   compact spectral field  S[c][ix][iy][iz]    ix,iy in [0,K), iz in [0,KZ)
+//       compact index i <-> signed wavenumber  i <= M ? i : i - K
+//       and full FFT index                     i <= M ? i : i + N - K
+//   after the x pass        A[f][x][iy][iz]     x in [0,N)
+//   after the y pass        B[f][x][y][iz]      y in [0,N)
+//   physical field          P[f][x][y][z]       z in [0,N)
+// Only modes inside the inclusive cube mask |k_i| <= N/3 are ever stored, so
+// the 2/3 rule is applied by construction and every pass reads/writes only
+// the kept part of its axis (zero padding lives in registers and LDS).
+#pragma once
+#include "tg_fft.hpp"
+
+namespace tg {
+
+template <int N>
+struct Geo {
+  static constexpr int M = N / 3;
+  static constexpr int K = 2 * M + 1;
+  static constexpr int KZ = M + 1;
+  __device__ static constexpr bool kept(int n) { return n <= M || n >= N - M; }
+  __device__ static constexpr int compact_of(int n) { return n <= M ? n : n - (N - K); }
+  __device__ static constexpr int full_of(int i) { return i <= M ? i : i + (N - K); }
+  __device__ static constexpr int kint(int i) { return i <= M ? i : i - K; }
+};
+
+// Launch shapes.  X/Y passes: L columns of kz per block, E points per thread.
+template <int N>
+struct CfgXY {
+  static constexpr int E = (N == 48 || N == 96) ? 12 : (N >= 128 ? 16 : (N == 4 ? 4 : 8));
+  static constexpr int L = 16;
+  static constexpr int NT = L * N / E;
+};
+// Z pass: E points per thread for the inverse, E/2 for the forward (which
+// has half as many sequences).  NB = 3 x lines per block for hydro.
+template <int N>
+struct CfgZ {
+  static constexpr int NB = (N >= 256 || N == 4) ? 12 : 24;
+  static constexpr int E = (N == 48 || N == 96) ? 24 : (N >= 128 ? 16 : (N == 4 ? 4 : 8));
+};
+// Forward-x stage pass: 24 lines per block (3 fields x 8 kz or 6 x 4).
+template <int N>
+struct CfgXF {
+  static constexpr int E = CfgXY<N>::E;
+  static constexpr int LINES = 24;
+  static constexpr int NT = LINES * N / E;
+};
+
+enum XMode { XM_HYDRO = 0, XM_MHD = 1, XM_COPY3 = 2, XM_DIV = 3, XM_CURL3 = 4 };
+enum ZMode { ZM_HYDRO = 0, ZM_MHD = 1, ZM_PHYS_OUT3 = 2, ZM_MAXABS = 3, ZM_PHYS_IN3 = 4, ZM_PHYS_OUT1 = 5 };
+enum FMode { FM_HYDRO_STAGE = 0, FM_MHD_STAGE = 1, FM_STORE3 = 2, FM_HYDRO_RHS = 3, FM_MHD_RHS = 4 };
+
+__host__ __device__ constexpr int xmode_fields(int m) {
+  return m == XM_HYDRO ? 6 : m == XM_MHD ? 12 : (m == XM_COPY3 || m == XM_CURL3) ? 3 : 1;
+}
+
+// ----------------------------------------------------------------- pass 1
+// Inverse transform along x of the fields derived from the compact state:
+// HYDRO -> (u, omega) = (S, i k x S); MHD -> (u, omega, b, j); COPY3 -> S;
+// CURL3 -> i k x S; DIV -> i k . S.  The 1/N^3 inverse normalisation is
+// applied here.
+template <typename R, int N, int MODE>
+__device__ __forceinline__ cpx<R> xfield(int f, const cpx<R>* __restrict__ su,
+                                         const cpx<R>* __restrict__ sb, size_t pt, size_t cs,
+                                         R kx, R ky, R kz) {
+  const cpx<R>* s = su;
+  if constexpr (MODE == XM_MHD) {
+    if (f >= 6) { s = sb; f -= 6; }
+  }
+  if constexpr (MODE == XM_CURL3) f += 3;
+  if (f < 3 && MODE != XM_DIV) return s[f * cs + pt];
+  const cpx<R> a = s[pt], b = s[cs + pt], c = s[2 * cs + pt];
+  if constexpr (MODE == XM_DIV) {
+    cpx<R> d = {kx * a.x + ky * b.x + kz * c.x, kx * a.y + ky * b.y + kz * c.y};
+    return cmuli(d);
+  }
+  cpx<R> r;
+  if (f == 3) r = {ky * c.x - kz * b.x, ky * c.y - kz * b.y};
+  else if (f == 4) r = {kz * a.x - kx * c.x, kz * a.y - kx * c.y};
+  else r = {kx * b.x - ky * a.x, kx * b.y - ky * a.y};
+  return cmuli(r);
+}
+
+template <typename R, int N, int MODE>
+__global__ __launch_bounds__(CfgXY<N>::NT) void k_inv_x(const cpx<R>* __restrict__ su,
+                                                        const cpx<R>* __restrict__ sb,
+                                                        cpx<R>* __restrict__ out,
+                                                        const cpx<R>* __restrict__ tw, R ks) {
+  using G = Geo<N>;
+  constexpr int L = CfgXY<N>::L, E = CfgXY<N>::E, NT = CfgXY<N>::NT;
+  constexpr int F = xmode_fields(MODE);
+  __shared__ cpx<R> buf[N * L];
+  const int iz0 = blockIdx.x * L, iy = blockIdx.y;
+  const int tid = threadIdx.x;
+  const size_t cs = size_t(G::K) * G::K * G::KZ;
+  const R scale = R(1) / (R(N) * R(N) * R(N));
+  const R ky = ks * R(G::kint(iy));
+  for (int f = 0; f < F; ++f) {
+    for (int idx = tid; idx < N * L; idx += NT) {
+      const int n = idx / L, ll = idx % L, iz = iz0 + ll;
+      cpx<R> v = {R(0), R(0)};
+      if (iz < G::KZ && G::kept(n)) {
+        const int ix = G::compact_of(n);
+        const size_t pt = (size_t(ix) * G::K + iy) * G::KZ + iz;
+        v = cscale(xfield<R, N, MODE>(f, su, sb, pt, cs, ks * R(G::kint(ix)), ky, ks * R(iz)), scale);
+      }
+      buf[n * L + ll] = v;
+    }
+    __syncthreads();
+    fft_lines<R, N, E, +1>(buf, L, L, tw);
+    for (int idx = tid; idx < N * L; idx += NT) {
+      const int n = idx / L, ll = idx % L, iz = iz0 + ll;
+      if (iz < G::KZ) out[((size_t(f) * N + n) * G::K + iy) * G::KZ + iz] = buf[n * L + ll];
+    }
+    __syncthreads();
+  }
+}
+
+// ----------------------------------------------------------------- pass 2/4
+// Transform along y for F fields.  DIR=+1: A[f][x][iy][iz] (K rows, zero
+// padded) -> B[f][x][y][iz].  DIR=-1: B -> A keeping the K masked rows.
+template <typename R, int N, int DIR>
+__global__ __launch_bounds__(CfgXY<N>::NT) void k_y(const cpx<R>* __restrict__ in,
+                                                    cpx<R>* __restrict__ out,
+                                                    const cpx<R>* __restrict__ tw) {
+  using G = Geo<N>;
+  constexpr int L = CfgXY<N>::L, E = CfgXY<N>::E, NT = CfgXY<N>::NT;
+  __shared__ cpx<R> buf[N * L];
+  const int iz0 = blockIdx.x * L, x = blockIdx.y, f = blockIdx.z;
+  const int tid = threadIdx.x;
+  const size_t plane = size_t(f) * N + x;
+  for (int idx = tid; idx < N * L; idx += NT) {
+    const int n = idx / L, ll = idx % L, iz = iz0 + ll;
+    cpx<R> v = {R(0), R(0)};
+    if (iz < G::KZ) {
+      if constexpr (DIR > 0) {
+        if (G::kept(n)) v = in[(plane * G::K + G::compact_of(n)) * G::KZ + iz];
+      } else {
+        v = in[(plane * N + n) * G::KZ + iz];
+      }
+    }
+    buf[n * L + ll] = v;
+  }
+  __syncthreads();
+  fft_lines<R, N, E, DIR>(buf, L, L, tw);
+  for (int idx = tid; idx < N * L; idx += NT) {
+    const int n = idx / L, ll = idx % L, iz = iz0 + ll;
+    if (iz >= G::KZ) continue;
+    if constexpr (DIR > 0) {
+      out[(plane * N + n) * G::KZ + iz] = buf[n * L + ll];
+    } else {
+      if (G::kept(n)) out[(plane * G::K + G::compact_of(n)) * G::KZ + iz] = buf[n * L + ll];
+    }
+  }
+}
+
+// ----------------------------------------------------------------- pass 3
+// Along z: complex-to-real of the inverse fields, the pointwise nonlinear
+// product, real-to-complex of the results, all for a group of lines
+// (x, y0..y0+LZ-1) held in LDS.  Two real fields share one complex sequence
+// (a + i b); Hermitian extension supplies the negative kz half.
+template <int MODE>
+struct ZShape;
+template <> struct ZShape<ZM_HYDRO> { static constexpr int SPL = 3, FIN = 6, FOUT = 3; };
+template <> struct ZShape<ZM_MHD> { static constexpr int SPL = 6, FIN = 12, FOUT = 6; };
+template <> struct ZShape<ZM_PHYS_OUT3> { static constexpr int SPL = 2, FIN = 3, FOUT = 0; };
+template <> struct ZShape<ZM_MAXABS> { static constexpr int SPL = 1, FIN = 1, FOUT = 0; };
+template <> struct ZShape<ZM_PHYS_IN3> { static constexpr int SPL = 3, FIN = 0, FOUT = 3; };
+template <> struct ZShape<ZM_PHYS_OUT1> { static constexpr int SPL = 1, FIN = 1, FOUT = 0; };
+
+// Per-mode block shape: LZ lines, NSEQ = LZ * SPL inverse-side sequences.
+template <int N, int MODE>
+struct CfgZM {
+  static constexpr int LZ = (MODE == ZM_MHD) ? CfgZ<N>::NB / 6 : CfgZ<N>::NB / 3;
+  static constexpr int NSEQ = LZ * ZShape<MODE>::SPL;
+  static constexpr int E = CfgZ<N>::E;
+  static constexpr int NT = NSEQ * N / E;
+  static constexpr int LP = NSEQ | 1;  // odd pitch: conflict-free column access
+};
+
+template <typename R>
+__device__ __forceinline__ void cross3(const R* a, const R* b, R* o) {
+  o[0] = a[1] * b[2] - a[2] * b[1];
+  o[1] = a[2] * b[0] - a[0] * b[2];
+  o[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+template <typename R, int N, int MODE>
+__global__ __launch_bounds__((CfgZM<N, MODE>::NT)) void k_z(const cpx<R>* __restrict__ in,
+                                                   cpx<R>* __restrict__ out,
+                                                   const R* __restrict__ phys_in,
+                                                   R* __restrict__ phys_out,
+                                                   const cpx<R>* __restrict__ tw, R lorentz) {
+  using G = Geo<N>;
+  using Z = CfgZM<N, MODE>;
+  using S = ZShape<MODE>;
+  constexpr int NSEQ = Z::NSEQ, LP = Z::LP, NT = Z::NT, E = Z::E;
+  constexpr int LZ = Z::LZ;  // lines per block
+  constexpr int KZ = G::KZ;
+  static_assert(N % LZ == 0, "lines per block must divide N");
+  __shared__ cpx<R> buf[N * LP];
+  const int tid = threadIdx.x;
+  const int x = blockIdx.y, y0 = blockIdx.x * LZ;
+  const size_t NNKZ = size_t(N) * N * KZ;
+  const size_t line0 = size_t(x) * N + y0;  // (x, y0) line index
+
+  if constexpr (MODE != ZM_PHYS_IN3) {
+    // ---- fill: sequence s = line*SPL + p packs fields (fa, fb)
+    for (int idx = tid; idx < NSEQ * N; idx += NT) {
+      const int s = idx / N, k = idx % N;
+      const int line = s / S::SPL, p = s % S::SPL;
+      int fa, fb;
+      if constexpr (MODE == ZM_HYDRO) { fa = p; fb = p + 3; }
+      else if constexpr (MODE == ZM_MHD) { fa = (p < 3) ? p : p + 3; fb = fa + 3; }
+      else if constexpr (MODE == ZM_PHYS_OUT3) { fa = 2 * p; fb = (2 * p + 1 < 3) ? 2 * p + 1 : -1; }
+      else { fa = 0; fb = -1; }
+      const bool neg = k > N / 2;
+      const int kz = neg ? N - k : k;
+      cpx<R> a = {R(0), R(0)}, b = {R(0), R(0)};
+      if (kz < KZ) {
+        const size_t off = (line0 + line) * KZ + kz;
+        a = in[fa * NNKZ + off];
+        if (fb >= 0) b = in[fb * NNKZ + off];
+        if (k == 0) { a.y = R(0); b.y = R(0); }
+        if (neg) { a.y = -a.y; b.y = -b.y; }
+      }
+      buf[k * LP + s] = {a.x - b.y, a.y + b.x};  // a + i b
+    }
+    __syncthreads();
+    fft_lines<R, N, E, +1>(buf, LP, NSEQ, tw);
+  }
+
+  if constexpr (MODE == ZM_PHYS_OUT3) {
+    for (int idx = tid; idx < LZ * 3 * N; idx += NT) {
+      const int z = idx % N, f = (idx / N) % 3, line = idx / (3 * N);
+      const cpx<R> v = buf[z * LP + line * S::SPL + f / 2];
+      phys_out[(size_t(f) * N * N + line0 + line) * N + z] = (f & 1) ? v.y : v.x;
+    }
+    return;
+  } else if constexpr (MODE == ZM_PHYS_OUT1) {
+    for (int idx = tid; idx < LZ * N; idx += NT) {
+      const int z = idx % N, line = idx / N;
+      phys_out[(line0 + line) * N + z] = buf[z * LP + line].x;
+    }
+    return;
+  } else if constexpr (MODE == ZM_MAXABS) {
+    __shared__ R red[NT];
+    R m = R(0);
+    for (int idx = tid; idx < LZ * N; idx += NT) {
+      const int z = idx % N, line = idx / N;
+      m = fmax(m, fabs(buf[z * LP + line].x));
+    }
+    red[tid] = m;
+    __syncthreads();
+    if (tid == 0) {
+      R mm = R(0);
+      for (int i = 0; i < NT; ++i) mm = fmax(mm, red[i]);
+      phys_out[size_t(blockIdx.y) * gridDim.x + blockIdx.x] = mm;
+    }
+    return;
+  } else {
+    // ---- products -> forward sequences.  Group g: HYDRO = 2 lines, 6 real
+    // outputs (n of line a, n of line b); MHD = 1 line, 6 real outputs
+    // (m = u x w + lorentz * j x b, e = u x b).  Each group fills 3 sequences
+    // q = 3g + {0,1,2} with pairs (r0,r1), (r2,r3), (r4,r5).
+    constexpr int NG = (MODE == ZM_MHD) ? LZ : LZ / 2;
+    constexpr int ITEMS = NG * N;
+    constexpr int IPT = (ITEMS + NT - 1) / NT;
+    R res[IPT][6];
+#pragma unroll
+    for (int it = 0; it < IPT; ++it) {
+      const int item = tid + it * NT;
+      if (item >= ITEMS) break;
+      const int z = item % N, g = item / N;
+      if constexpr (MODE == ZM_HYDRO) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int line = 2 * g + h;
+          R u[3], w[3];
+#pragma unroll
+          for (int p = 0; p < 3; ++p) {
+            const cpx<R> v = buf[z * LP + line * 3 + p];
+            u[p] = v.x; w[p] = v.y;
+          }
+          cross3(u, w, &res[it][3 * h]);
+        }
+      } else if constexpr (MODE == ZM_MHD) {
+        R u[3], w[3], b[3], j[3];
+#pragma unroll
+        for (int p = 0; p < 3; ++p) {
+          const cpx<R> v = buf[z * LP + g * 6 + p];
+          const cpx<R> q = buf[z * LP + g * 6 + 3 + p];
+          u[p] = v.x; w[p] = v.y; b[p] = q.x; j[p] = q.y;
+        }
+        R uw[3], jb[3];
+        cross3(u, w, uw);
+        cross3(j, b, jb);
+#pragma unroll
+        for (int p = 0; p < 3; ++p) res[it][p] = uw[p] + lorentz * jb[p];
+        cross3(u, b, &res[it][3]);
+      } else {  // PHYS_IN3: lines (2g, 2g+1), fields u,v,w read from physical
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const size_t line = line0 + 2 * g + h;
+#pragma unroll
+          for (int p = 0; p < 3; ++p) res[it][3 * h + p] = phys_in[(p * size_t(N) * N + line) * N + z];
+        }
+      }
+    }
+    if constexpr (MODE != ZM_PHYS_IN3) __syncthreads();
+    // Pack: HYDRO/PHYS_IN: (nx_a + i ny_a), (nx_b + i ny_b), (nz_a + i nz_b);
+    //       MHD:           (m_x + i m_y), (m_z + i e_x), (e_y + i e_z).
+#pragma unroll
+    for (int it = 0; it < IPT; ++it) {
+      const int item = tid + it * NT;
+      if (item >= ITEMS) break;
+      const int z = item % N, g = item / N;
+      const R* r = res[it];
+      if constexpr (MODE == ZM_MHD) {
+        buf[z * LP + 3 * g + 0] = {r[0], r[1]};
+        buf[z * LP + 3 * g + 1] = {r[2], r[3]};
+        buf[z * LP + 3 * g + 2] = {r[4], r[5]};
+      } else {
+        buf[z * LP + 3 * g + 0] = {r[0], r[1]};
+        buf[z * LP + 3 * g + 1] = {r[3], r[4]};
+        buf[z * LP + 3 * g + 2] = {r[2], r[5]};
+      }
+    }
+    __syncthreads();
+    fft_lines<R, N, E / 2, -1>(buf, LP, NSEQ / 2, tw);
+    // ---- unpack the kept kz of each pair and store
+    constexpr int NQ = NSEQ / 2;
+    for (int idx = tid; idx < NQ * KZ; idx += NT) {
+      const int kz = idx % KZ, q = idx / KZ;
+      const cpx<R> P = buf[kz * LP + q];
+      const cpx<R> Qc = buf[((N - kz) % N) * LP + q];  // conj taken below
+      const cpx<R> re = {R(0.5) * (P.x + Qc.x), R(0.5) * (P.y - Qc.y)};
+      const cpx<R> im = {R(0.5) * (P.y + Qc.y), R(0.5) * (Qc.x - P.x)};
+      const int g = q / 3, p = q % 3;
+      int line_a, fa, line_b, fb;
+      if constexpr (MODE == ZM_MHD) {
+        line_a = line_b = g;
+        fa = 2 * p; fb = 2 * p + 1;
+      } else {
+        if (p < 2) { line_a = line_b = 2 * g + p; fa = 0; fb = 1; }
+        else { line_a = 2 * g; line_b = 2 * g + 1; fa = 2; fb = 2; }
+      }
+      out[fa * NNKZ + (line0 + line_a) * KZ + kz] = re;
+      out[fb * NNKZ + (line0 + line_b) * KZ + kz] = im;
+    }
+  }
+}
+
+// ----------------------------------------------------------------- pass 5
+// Forward transform along x of FF fields (3 or 6) for one (iy, kz tile),
+// keeping the K masked rows, followed by the spectral epilogue: Leray
+// projection, viscous/resistive term and the RK4 stage update (or a plain
+// store of the transform / of the right-hand side).
+template <typename R>
+struct StageArgs {
+  const cpx<R>* u0;   // state at the start of the step (u) / (b)
+  const cpx<R>* b0;
+  const cpx<R>* us;   // stage input
+  const cpx<R>* bs;
+  cpx<R>* unext;      // next stage input (stages 0-2) ; u0 itself at stage 3
+  cpx<R>* bnext;
+  cpx<R>* uacc;       // r1 + 2 r2 + 2 r3 accumulator
+  cpx<R>* bacc;
+  R dt, nu, eta, ks;
+  int stage;
+};
+
+template <typename R>
+__device__ __forceinline__ void project(R kx, R ky, R kz, R k2s, cpx<R>* n) {
+  const cpx<R> kd = {kx * n[0].x + ky * n[1].x + kz * n[2].x, kx * n[0].y + ky * n[1].y + kz * n[2].y};
+  n[0] = {n[0].x - kx * kd.x / k2s, n[0].y - kx * kd.y / k2s};
+  n[1] = {n[1].x - ky * kd.x / k2s, n[1].y - ky * kd.y / k2s};
+  n[2] = {n[2].x - kz * kd.x / k2s, n[2].y - kz * kd.y / k2s};
+}
+
+// RK4 bookkeeping shared by the u and b halves of the state.
+template <typename R>
+__device__ __forceinline__ void rk4_update(int stage, R dt, size_t pt, size_t cs, const cpx<R>* r,
+                                           const cpx<R>* x0, cpx<R>* xnext, cpx<R>* acc) {
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const size_t o = c * cs + pt;
+    const cpx<R> a0 = x0[o];
+    if (stage == 0) {
+      acc[o] = r[c];
+      const R h = R(0.5) * dt;
+      xnext[o] = {a0.x + h * r[c].x, a0.y + h * r[c].y};
+    } else if (stage < 3) {
+      const cpx<R> s = acc[o];
+      acc[o] = {s.x + R(2) * r[c].x, s.y + R(2) * r[c].y};
+      const R h = (stage == 1) ? R(0.5) * dt : dt;
+      xnext[o] = {a0.x + h * r[c].x, a0.y + h * r[c].y};
+    } else {
+      const cpx<R> s = acc[o];
+      const R h = dt / R(6);
+      xnext[o] = {a0.x + h * (s.x + r[c].x), a0.y + h * (s.y + r[c].y)};
+    }
+  }
+}
+
+template <typename R, int N, int MODE>
+__global__ __launch_bounds__(CfgXF<N>::NT) void k_fwd_x(const cpx<R>* __restrict__ in,
+                                                        StageArgs<R> a,
+                                                        const cpx<R>* __restrict__ tw) {
+  using G = Geo<N>;
+  constexpr int FF = (MODE == FM_MHD_STAGE || MODE == FM_MHD_RHS) ? 6 : 3;
+  constexpr int LINES = CfgXF<N>::LINES, NT = CfgXF<N>::NT, E = CfgXF<N>::E;
+  constexpr int L = LINES / FF;
+  constexpr int LP = LINES;
+  __shared__ cpx<R> buf[N * LP];
+  const int iz0 = blockIdx.x * L, iy = blockIdx.y;
+  const int tid = threadIdx.x;
+  for (int idx = tid; idx < FF * N * L; idx += NT) {
+    const int ll = idx % L, n = (idx / L) % N, f = idx / (L * N);
+    const int iz = iz0 + ll;
+    cpx<R> v = {R(0), R(0)};
+    if (iz < G::KZ) v = in[((size_t(f) * N + n) * G::K + iy) * G::KZ + iz];
+    buf[n * LP + f * L + ll] = v;
+  }
+  __syncthreads();
+  fft_lines<R, N, E, -1>(buf, LP, LINES, tw);
+  const size_t cs = size_t(G::K) * G::K * G::KZ;
+  const R ky = a.ks * R(G::kint(iy));
+  for (int idx = tid; idx < G::K * L; idx += NT) {
+    const int ll = idx % L, ix = idx / L, iz = iz0 + ll;
+    if (iz >= G::KZ) continue;
+    const int n = G::full_of(ix);
+    const size_t pt = (size_t(ix) * G::K + iy) * G::KZ + iz;
+    cpx<R> v[FF];
+#pragma unroll
+    for (int f = 0; f < FF; ++f) v[f] = buf[n * LP + f * L + ll];
+    if constexpr (MODE == FM_STORE3) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) a.unext[c * cs + pt] = v[c];
+      continue;
+    } else {
+      const R kx = a.ks * R(G::kint(ix)), kz = a.ks * R(iz);
+      const R k2 = kx * kx + ky * ky + kz * kz;
+      const R k2s = (k2 == R(0)) ? R(1) : k2;
+      // momentum: r_u = P(u x w [+ lorentz j x b]) - nu k2 u_s
+      project(kx, ky, kz, k2s, v);
+      cpx<R> ru[3];
+      const R nk = a.nu * k2;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const cpx<R> s = a.us[c * cs + pt];
+        ru[c] = {v[c].x - nk * s.x, v[c].y - nk * s.y};
+      }
+      if constexpr (FF == 6) {
+        // induction: r_b = P(i k x e) - eta k2 b_s
+        const cpx<R> e0 = v[3], e1 = v[4], e2 = v[5];
+        cpx<R> cb[3];
+        cb[0] = cmuli(cpx<R>{ky * e2.x - kz * e1.x, ky * e2.y - kz * e1.y});
+        cb[1] = cmuli(cpx<R>{kz * e0.x - kx * e2.x, kz * e0.y - kx * e2.y});
+        cb[2] = cmuli(cpx<R>{kx * e1.x - ky * e0.x, kx * e1.y - ky * e0.y});
+        project(kx, ky, kz, k2s, cb);
+        const R ek = a.eta * k2;
+        cpx<R> rb[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const cpx<R> s = a.bs[c * cs + pt];
+          rb[c] = {cb[c].x - ek * s.x, cb[c].y - ek * s.y};
+        }
+        if constexpr (MODE == FM_MHD_RHS) {
+#pragma unroll
+          for (int c = 0; c < 3; ++c) { a.unext[c * cs + pt] = ru[c]; a.bnext[c * cs + pt] = rb[c]; }
+        } else {
+          rk4_update(a.stage, a.dt, pt, cs, ru, a.u0, a.unext, a.uacc);
+          rk4_update(a.stage, a.dt, pt, cs, rb, a.b0, a.bnext, a.bacc);
+        }
+      } else {
+        if constexpr (MODE == FM_HYDRO_RHS) {
+#pragma unroll
+          for (int c = 0; c < 3; ++c) a.unext[c * cs + pt] = ru[c];
+        } else {
+          rk4_update(a.stage, a.dt, pt, cs, ru, a.u0, a.unext, a.uacc);
+        }
+      }
+    }
+  }
+}
+
+// ----------------------------------------------------------------- diagnostics
+// Sums over the compact half spectrum with Hermitian weight (1 at kz=0, 2
+// otherwise), accumulated in FP64 with Neumaier compensation.  Slots:
+//  0 sum|u|^2  1 sum|k x u|^2  2 sum k^2|u|^2  3 sum|b|^2  4 sum Re(u.conj b)
+//  5 sum|k.u|^2  6 sum|k.b|^2  7 sum|k x b|^2
+constexpr int NDIAG = 8;
+constexpr int DIAG_NT = 256;
+
+struct Neu {
+  double s = 0.0, c = 0.0;
+  __device__ void add(double v) {
+    const double t = s + v;
+    if (fabs(s) >= fabs(v)) c += (s - t) + v;
+    else c += (v - t) + s;
+    s = t;
+  }
+  __device__ double val() const { return s + c; }
+};
+
+template <typename R, int N>
+__global__ __launch_bounds__(DIAG_NT) void k_diag(const cpx<R>* __restrict__ su,
+                                                  const cpx<R>* __restrict__ sb,
+                                                  double* __restrict__ partial, double ks) {
+  using G = Geo<N>;
+  const size_t cs = size_t(G::K) * G::K * G::KZ;
+  Neu acc[NDIAG];
+  for (size_t pt = blockIdx.x * size_t(DIAG_NT) + threadIdx.x; pt < cs; pt += size_t(gridDim.x) * DIAG_NT) {
+    const int iz = int(pt % G::KZ);
+    const int iy = int((pt / G::KZ) % G::K);
+    const int ix = int(pt / (size_t(G::KZ) * G::K));
+    const double w = iz == 0 ? 1.0 : 2.0;
+    const double kx = ks * G::kint(ix), ky = ks * G::kint(iy), kz = ks * iz;
+    const double k2 = kx * kx + ky * ky + kz * kz;
+    double ur[3], ui[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) { ur[c] = double(su[c * cs + pt].x); ui[c] = double(su[c * cs + pt].y); }
+    const double uu = ur[0] * ur[0] + ui[0] * ui[0] + ur[1] * ur[1] + ui[1] * ui[1] + ur[2] * ur[2] + ui[2] * ui[2];
+    auto curl2 = [&](const double* r, const double* i) {
+      const double ar = ky * r[2] - kz * r[1], ai = ky * i[2] - kz * i[1];
+      const double br = kz * r[0] - kx * r[2], bi = kz * i[0] - kx * i[2];
+      const double cr = kx * r[1] - ky * r[0], ci = kx * i[1] - ky * i[0];
+      return ar * ar + ai * ai + br * br + bi * bi + cr * cr + ci * ci;
+    };
+    auto div2 = [&](const double* r, const double* i) {
+      const double dr = kx * r[0] + ky * r[1] + kz * r[2], di = kx * i[0] + ky * i[1] + kz * i[2];
+      return dr * dr + di * di;
+    };
+    acc[0].add(w * uu);
+    acc[1].add(w * curl2(ur, ui));
+    acc[2].add(w * k2 * uu);
+    acc[5].add(w * div2(ur, ui));
+    if (sb) {
+      double br[3], bi[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) { br[c] = double(sb[c * cs + pt].x); bi[c] = double(sb[c * cs + pt].y); }
+      acc[3].add(w * (br[0] * br[0] + bi[0] * bi[0] + br[1] * br[1] + bi[1] * bi[1] + br[2] * br[2] + bi[2] * bi[2]));
+      acc[4].add(w * (ur[0] * br[0] + ui[0] * bi[0] + ur[1] * br[1] + ui[1] * bi[1] + ur[2] * br[2] + ui[2] * bi[2]));
+      acc[6].add(w * div2(br, bi));
+      acc[7].add(w * curl2(br, bi));
+    }
+  }
+  __shared__ double red[DIAG_NT];
+  for (int q = 0; q < NDIAG; ++q) {
+    red[threadIdx.x] = acc[q].val();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      Neu s;
+      for (int i = 0; i < DIAG_NT; ++i) s.add(red[i]);
+      partial[size_t(blockIdx.x) * NDIAG + q] = s.val();
+    }
+    __syncthreads();
+  }
+}
+
+// Final reduction: out[0..NDIAG) = compensated sums of the partials,
+// out[NDIAG] = max of `nmax` maxima (u divergence), out[NDIAG+1] = (b).
+template <typename R>
+__global__ __launch_bounds__(64) void k_diag_final(const double* __restrict__ partial, int nblk,
+                                                   const R* __restrict__ maxu, const R* __restrict__ maxb,
+                                                   int nmax, double* __restrict__ out) {
+  const int q = threadIdx.x;
+  if (q < NDIAG) {
+    Neu s;
+    for (int b = 0; b < nblk; ++b) s.add(partial[size_t(b) * NDIAG + q]);
+    out[q] = s.val();
+  } else if (q == NDIAG || q == NDIAG + 1) {
+    const R* m = (q == NDIAG) ? maxu : maxb;
+    double mm = 0.0;
+    if (m)
+      for (int i = 0; i < nmax; ++i) mm = fmax(mm, double(m[i]));
+    out[q] = mm;
+  }
+}
+
+}  // namespace tg

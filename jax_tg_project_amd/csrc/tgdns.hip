@@ -1,0 +1,483 @@
+// libtgdns.so: host side of the C ABI declared in include/tgdns.h.
+// Plans own twiddle tables; callers own every other buffer.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/tgdns.h"
+#include "tg_passes.hpp"
+
+using namespace tg;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define TG_HIP(expr)                                                                      \
+  do {                                                                                    \
+    hipError_t e_ = (expr);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      return fail(TG_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));        \
+  } while (0)
+
+#define TG_LAUNCHED()                                                                     \
+  do {                                                                                    \
+    hipError_t e_ = hipGetLastError();                                                    \
+    if (e_ != hipSuccess)                                                                 \
+      return fail(TG_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e_));   \
+  } while (0)
+
+constexpr int kSizes[] = {4, 8, 16, 32, 48, 64, 96, 128, 256};
+
+size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+}  // namespace
+
+struct tg_plan {
+  int n, prec, mhd;
+  int M, K, KZ;
+  double ks;          // 2 pi / L
+  void* tw;           // N complex twiddles exp(-2 pi i j / N) in plan precision
+  size_t cb;          // bytes per complex element
+  size_t compact;     // complex elements per compact component
+  // workspace layout (byte offsets)
+  size_t offA, offB, offU1, offAcc, offPart, offMax, offOut, wsBytes;
+  int nDiagBlk, nMax;
+};
+
+namespace {
+
+template <typename R>
+struct Ptrs {
+  cpx<R>* A;
+  cpx<R>* B;
+  cpx<R>* U1;    // stage input u (and b right after it for MHD)
+  cpx<R>* Acc;   // accumulator u (and b)
+  double* part;
+  R* maxu;
+  R* maxb;
+  double* out;
+};
+
+template <typename R>
+Ptrs<R> carve(const tg_plan* p, void* ws) {
+  char* w = static_cast<char*>(ws);
+  Ptrs<R> q;
+  q.A = reinterpret_cast<cpx<R>*>(w + p->offA);
+  q.B = reinterpret_cast<cpx<R>*>(w + p->offB);
+  q.U1 = reinterpret_cast<cpx<R>*>(w + p->offU1);
+  q.Acc = reinterpret_cast<cpx<R>*>(w + p->offAcc);
+  q.part = reinterpret_cast<double*>(w + p->offPart);
+  q.maxu = reinterpret_cast<R*>(w + p->offMax);
+  q.maxb = q.maxu + p->nMax;
+  q.out = reinterpret_cast<double*>(w + p->offOut);
+  return q;
+}
+
+inline int cdiv(int a, int b) { return (a + b - 1) / b; }
+
+// ---------------------------------------------------------------- launchers
+template <typename R, int N>
+struct Launch {
+  using G = Geo<N>;
+  static int inv_x(int mode, const cpx<R>* su, const cpx<R>* sb, cpx<R>* out, const cpx<R>* tw, R ks,
+                   hipStream_t s) {
+    dim3 grid(cdiv(G::KZ, CfgXY<N>::L), G::K), blk(CfgXY<N>::NT);
+    switch (mode) {
+      case XM_HYDRO: hipLaunchKernelGGL((k_inv_x<R, N, XM_HYDRO>), grid, blk, 0, s, su, sb, out, tw, ks); break;
+      case XM_MHD: hipLaunchKernelGGL((k_inv_x<R, N, XM_MHD>), grid, blk, 0, s, su, sb, out, tw, ks); break;
+      case XM_COPY3: hipLaunchKernelGGL((k_inv_x<R, N, XM_COPY3>), grid, blk, 0, s, su, sb, out, tw, ks); break;
+      case XM_CURL3: hipLaunchKernelGGL((k_inv_x<R, N, XM_CURL3>), grid, blk, 0, s, su, sb, out, tw, ks); break;
+      default: hipLaunchKernelGGL((k_inv_x<R, N, XM_DIV>), grid, blk, 0, s, su, sb, out, tw, ks); break;
+    }
+    TG_LAUNCHED();
+    return TG_OK;
+  }
+  static int y(int dir, int nf, const cpx<R>* in, cpx<R>* out, const cpx<R>* tw, hipStream_t s) {
+    dim3 grid(cdiv(G::KZ, CfgXY<N>::L), N, nf), blk(CfgXY<N>::NT);
+    if (dir > 0) hipLaunchKernelGGL((k_y<R, N, +1>), grid, blk, 0, s, in, out, tw);
+    else hipLaunchKernelGGL((k_y<R, N, -1>), grid, blk, 0, s, in, out, tw);
+    TG_LAUNCHED();
+    return TG_OK;
+  }
+  template <int MODE>
+  static int z_(const cpx<R>* in, cpx<R>* out, const R* pin, R* pout, const cpx<R>* tw, R lor, hipStream_t s) {
+    using Z = CfgZM<N, MODE>;
+    dim3 grid(N / Z::LZ, N), blk(Z::NT);
+    hipLaunchKernelGGL((k_z<R, N, MODE>), grid, blk, 0, s, in, out, pin, pout, tw, lor);
+    TG_LAUNCHED();
+    return TG_OK;
+  }
+  static int z(int mode, const cpx<R>* in, cpx<R>* out, const R* pin, R* pout, const cpx<R>* tw, R lor,
+               hipStream_t s) {
+    switch (mode) {
+      case ZM_HYDRO: return z_<ZM_HYDRO>(in, out, pin, pout, tw, lor, s);
+      case ZM_MHD: return z_<ZM_MHD>(in, out, pin, pout, tw, lor, s);
+      case ZM_PHYS_OUT3: return z_<ZM_PHYS_OUT3>(in, out, pin, pout, tw, lor, s);
+      case ZM_MAXABS: return z_<ZM_MAXABS>(in, out, pin, pout, tw, lor, s);
+      case ZM_PHYS_OUT1: return z_<ZM_PHYS_OUT1>(in, out, pin, pout, tw, lor, s);
+      default: return z_<ZM_PHYS_IN3>(in, out, pin, pout, tw, lor, s);
+    }
+  }
+  static int nmax() { return (N / CfgZM<N, ZM_MAXABS>::LZ) * N; }
+  static int fwd_x(int mode, const cpx<R>* in, const StageArgs<R>& a, const cpx<R>* tw, hipStream_t s) {
+    const int ff = (mode == FM_MHD_STAGE || mode == FM_MHD_RHS) ? 6 : 3;
+    const int L = CfgXF<N>::LINES / ff;
+    dim3 grid(cdiv(G::KZ, L), G::K), blk(CfgXF<N>::NT);
+    switch (mode) {
+      case FM_HYDRO_STAGE: hipLaunchKernelGGL((k_fwd_x<R, N, FM_HYDRO_STAGE>), grid, blk, 0, s, in, a, tw); break;
+      case FM_MHD_STAGE: hipLaunchKernelGGL((k_fwd_x<R, N, FM_MHD_STAGE>), grid, blk, 0, s, in, a, tw); break;
+      case FM_STORE3: hipLaunchKernelGGL((k_fwd_x<R, N, FM_STORE3>), grid, blk, 0, s, in, a, tw); break;
+      case FM_HYDRO_RHS: hipLaunchKernelGGL((k_fwd_x<R, N, FM_HYDRO_RHS>), grid, blk, 0, s, in, a, tw); break;
+      default: hipLaunchKernelGGL((k_fwd_x<R, N, FM_MHD_RHS>), grid, blk, 0, s, in, a, tw); break;
+    }
+    TG_LAUNCHED();
+    return TG_OK;
+  }
+  static int diag(const cpx<R>* su, const cpx<R>* sb, double* part, int nblk, double ks, hipStream_t s) {
+    hipLaunchKernelGGL((k_diag<R, N>), dim3(nblk), dim3(DIAG_NT), 0, s, su, sb, part, ks);
+    TG_LAUNCHED();
+    return TG_OK;
+  }
+};
+
+#define TG_RET(...)          \
+  do {                       \
+    int r_ = (__VA_ARGS__);  \
+    if (r_) return r_;       \
+  } while (0)
+
+// One right-hand side: stage input (us, bs) -> z-pass input in A/B -> the
+// forward-x epilogue described by `mode` / `a`.
+template <typename R, int N>
+int rhs_passes(const tg_plan* p, const Ptrs<R>& w, const cpx<R>* us, const cpx<R>* bs, int fmode,
+               const StageArgs<R>& a, R lor, hipStream_t s) {
+  using Lc = Launch<R, N>;
+  const cpx<R>* tw = static_cast<const cpx<R>*>(p->tw);
+  const bool mhd = p->mhd != 0;
+  const int fi = mhd ? 12 : 6, fo = mhd ? 6 : 3;
+  TG_RET(Lc::inv_x(mhd ? XM_MHD : XM_HYDRO, us, bs, w.A, tw, R(p->ks), s));
+  TG_RET(Lc::y(+1, fi, w.A, w.B, tw, s));
+  TG_RET(Lc::z(mhd ? ZM_MHD : ZM_HYDRO, w.B, w.A, nullptr, nullptr, tw, lor, s));
+  TG_RET(Lc::y(-1, fo, w.A, w.B, tw, s));
+  TG_RET(Lc::fwd_x(fmode, w.B, a, tw, s));
+  return TG_OK;
+}
+
+template <typename R, int N>
+struct Ops {
+  static int forward(const tg_plan* p, const R* phys, cpx<R>* spec, void* ws, hipStream_t s) {
+    using Lc = Launch<R, N>;
+    Ptrs<R> w = carve<R>(p, ws);
+    const cpx<R>* tw = static_cast<const cpx<R>*>(p->tw);
+    TG_RET(Lc::z(ZM_PHYS_IN3, nullptr, w.A, phys, nullptr, tw, R(0), s));
+    TG_RET(Lc::y(-1, 3, w.A, w.B, tw, s));
+    StageArgs<R> a{};
+    a.unext = spec;
+    a.ks = R(p->ks);
+    return Lc::fwd_x(FM_STORE3, w.B, a, tw, s);
+  }
+  static int inverse(const tg_plan* p, const cpx<R>* spec, int op, R* phys, void* ws, hipStream_t s) {
+    using Lc = Launch<R, N>;
+    Ptrs<R> w = carve<R>(p, ws);
+    const cpx<R>* tw = static_cast<const cpx<R>*>(p->tw);
+    const int xm = op == TG_OP_CURL ? XM_CURL3 : op == TG_OP_DIV ? XM_DIV : XM_COPY3;
+    const int nf = op == TG_OP_DIV ? 1 : 3;
+    TG_RET(Lc::inv_x(xm, spec, nullptr, w.A, tw, R(p->ks), s));
+    TG_RET(Lc::y(+1, nf, w.A, w.B, tw, s));
+    return Lc::z(op == TG_OP_DIV ? ZM_PHYS_OUT1 : ZM_PHYS_OUT3, w.B, nullptr, nullptr, phys, tw, R(0), s);
+  }
+  static int rhs(const tg_plan* p, const cpx<R>* u, const cpx<R>* b, double nu, double eta, double lor,
+                 cpx<R>* ru, cpx<R>* rb, void* ws, hipStream_t s) {
+    Ptrs<R> w = carve<R>(p, ws);
+    StageArgs<R> a{};
+    a.us = u; a.bs = b; a.unext = ru; a.bnext = rb;
+    a.nu = R(nu); a.eta = R(eta); a.ks = R(p->ks);
+    return rhs_passes<R, N>(p, w, u, b, p->mhd ? FM_MHD_RHS : FM_HYDRO_RHS, a, R(lor), s);
+  }
+  static int rk4(const tg_plan* p, cpx<R>* u, cpx<R>* b, int nsteps, double dt, double nu, double eta,
+                 double lor, void* ws, hipStream_t s) {
+    Ptrs<R> w = carve<R>(p, ws);
+    const size_t vec = 3 * p->compact;
+    cpx<R>* u1 = w.U1;
+    cpx<R>* b1 = w.U1 + vec;
+    cpx<R>* ua = w.Acc;
+    cpx<R>* ba = w.Acc + vec;
+    const int fm = p->mhd ? FM_MHD_STAGE : FM_HYDRO_STAGE;
+    for (int step = 0; step < nsteps; ++step) {
+      for (int st = 0; st < 4; ++st) {
+        StageArgs<R> a{};
+        a.u0 = u; a.b0 = b;
+        a.us = st == 0 ? u : u1;
+        a.bs = st == 0 ? b : b1;
+        a.unext = st == 3 ? u : u1;
+        a.bnext = st == 3 ? b : b1;
+        a.uacc = ua; a.bacc = ba;
+        a.dt = R(dt); a.nu = R(nu); a.eta = R(eta); a.ks = R(p->ks);
+        a.stage = st;
+        TG_RET(rhs_passes<R, N>(p, w, a.us, a.bs, fm, a, R(lor), s));
+      }
+    }
+    return TG_OK;
+  }
+  static int diagnostics(const tg_plan* p, const cpx<R>* u, const cpx<R>* b, int divmax, double* out,
+                         void* ws, hipStream_t s) {
+    using Lc = Launch<R, N>;
+    Ptrs<R> w = carve<R>(p, ws);
+    const cpx<R>* tw = static_cast<const cpx<R>*>(p->tw);
+    TG_RET(Lc::diag(u, b, w.part, p->nDiagBlk, p->ks, s));
+    int nm = 0;
+    if (divmax) {
+      nm = p->nMax;
+      TG_RET(Lc::inv_x(XM_DIV, u, nullptr, w.A, tw, R(p->ks), s));
+      TG_RET(Lc::y(+1, 1, w.A, w.B, tw, s));
+      TG_RET(Lc::z(ZM_MAXABS, w.B, nullptr, nullptr, w.maxu, tw, R(0), s));
+      if (b) {
+        TG_RET(Lc::inv_x(XM_DIV, b, nullptr, w.A, tw, R(p->ks), s));
+        TG_RET(Lc::y(+1, 1, w.A, w.B, tw, s));
+        TG_RET(Lc::z(ZM_MAXABS, w.B, nullptr, nullptr, w.maxb, tw, R(0), s));
+      }
+    }
+    hipLaunchKernelGGL((k_diag_final<R>), dim3(1), dim3(64), 0, s, w.part, p->nDiagBlk,
+                       divmax ? w.maxu : nullptr, (divmax && b) ? w.maxb : nullptr, nm, w.out);
+    TG_LAUNCHED();
+    double host[NDIAG + 2];
+    TG_HIP(hipMemcpyAsync(host, w.out, sizeof(host), hipMemcpyDeviceToHost, s));
+    TG_HIP(hipStreamSynchronize(s));
+    const double n3 = double(p->n) * p->n * p->n;
+    const double n6 = n3 * n3;
+    out[0] = 0.5 * host[0] / n6;
+    out[1] = 0.5 * host[1] / n6;
+    out[2] = host[2] / n6;
+    out[3] = 0.5 * host[3] / n6;
+    out[4] = host[4] / n6;
+    out[5] = host[5] / n6;
+    out[6] = host[6] / n6;
+    out[7] = host[7] / n6;
+    out[8] = host[8];
+    out[9] = host[9];
+    return TG_OK;
+  }
+};
+
+// Runtime N -> template dispatch.
+template <typename R, template <typename, int> class Fn>
+struct Dispatch {
+  template <typename Call>
+  static int run(int n, Call&& c) {
+    switch (n) {
+      case 4: return c(Fn<R, 4>{});
+      case 8: return c(Fn<R, 8>{});
+      case 16: return c(Fn<R, 16>{});
+      case 32: return c(Fn<R, 32>{});
+      case 48: return c(Fn<R, 48>{});
+      case 64: return c(Fn<R, 64>{});
+      case 96: return c(Fn<R, 96>{});
+      case 128: return c(Fn<R, 128>{});
+      case 256: return c(Fn<R, 256>{});
+      default: return fail(TG_ERR_UNSUPPORTED, "grid size N=" + std::to_string(n) + " has no compiled plan");
+    }
+  }
+};
+
+template <typename Call>
+int with_plan(const tg_plan* p, Call&& c) {
+  if (p->prec == 64) return Dispatch<double, Ops>::run(p->n, [&](auto ops) { return c(ops, (double*)nullptr); });
+  return Dispatch<float, Ops>::run(p->n, [&](auto ops) { return c(ops, (float*)nullptr); });
+}
+
+int check_plan(const tg_plan* p) {
+  if (!p) return fail(TG_ERR_ARG, "plan is NULL");
+  return TG_OK;
+}
+
+template <int N>
+int nmax_for() { return Launch<float, N>::nmax(); }
+
+int nmax_rt(int n) {
+  switch (n) {
+    case 4: return nmax_for<4>();
+    case 8: return nmax_for<8>();
+    case 16: return nmax_for<16>();
+    case 32: return nmax_for<32>();
+    case 48: return nmax_for<48>();
+    case 64: return nmax_for<64>();
+    case 96: return nmax_for<96>();
+    case 128: return nmax_for<128>();
+    case 256: return nmax_for<256>();
+  }
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tg_version(void) { return 1; }
+
+int tg_supported_sizes(int* out, int cap) {
+  const int n = int(sizeof(kSizes) / sizeof(kSizes[0]));
+  for (int i = 0; i < n && i < cap; ++i) out[i] = kSizes[i];
+  return n;
+}
+
+int tg_plan_create(int n, int precision_bits, int mhd, double domain_length, tg_plan** out) {
+  if (!out) return fail(TG_ERR_ARG, "out is NULL");
+  *out = nullptr;
+  bool ok = false;
+  for (int s : kSizes) ok |= (s == n);
+  if (!ok) return fail(TG_ERR_UNSUPPORTED, "grid size N=" + std::to_string(n) + " has no compiled plan");
+  if (precision_bits != 64 && precision_bits != 32)
+    return fail(TG_ERR_ARG, "precision_bits must be 64 or 32");
+  if (!(domain_length > 0.0)) return fail(TG_ERR_ARG, "domain_length must be positive");
+  tg_plan* p = new tg_plan{};
+  p->n = n;
+  p->prec = precision_bits;
+  p->mhd = mhd ? 1 : 0;
+  p->M = n / 3;
+  p->K = 2 * p->M + 1;
+  p->KZ = p->M + 1;
+  p->ks = 2.0 * M_PI / domain_length;
+  if (domain_length == 2.0 * M_PI) p->ks = 1.0;
+  p->cb = precision_bits == 64 ? 16 : 8;
+  p->compact = size_t(p->K) * p->K * p->KZ;
+  const size_t N = n;
+  const size_t fi = p->mhd ? 12 : 6;
+  const size_t sA = fi * N * p->K * p->KZ * p->cb;
+  const size_t sB = fi * N * N * p->KZ * p->cb;
+  const size_t sU = (p->mhd ? 2 : 1) * 3 * p->compact * p->cb;
+  p->nDiagBlk = 512;
+  p->nMax = nmax_rt(n);
+  size_t o = 0;
+  p->offA = o; o += align256(sA);
+  p->offB = o; o += align256(sB);
+  p->offU1 = o; o += align256(sU);
+  p->offAcc = o; o += align256(sU);
+  p->offPart = o; o += align256(size_t(p->nDiagBlk) * NDIAG * sizeof(double));
+  p->offMax = o; o += align256(2 * size_t(p->nMax) * (p->cb / 2));
+  p->offOut = o; o += align256((NDIAG + 2) * sizeof(double));
+  p->wsBytes = o;
+  // twiddles exp(-2 pi i j / N), evaluated in long double on the host
+  std::vector<char> host(N * p->cb);
+  for (size_t j = 0; j < N; ++j) {
+    const long double ang = -2.0L * 3.14159265358979323846264338327950288L * (long double)j / (long double)N;
+    const long double c = cosl(ang), sn = sinl(ang);
+    if (p->prec == 64) {
+      double v[2] = {double(c), double(sn)};
+      memcpy(host.data() + j * 16, v, 16);
+    } else {
+      float v[2] = {float(c), float(sn)};
+      memcpy(host.data() + j * 8, v, 8);
+    }
+  }
+  hipError_t e = hipMalloc(&p->tw, host.size());
+  if (e != hipSuccess) {
+    delete p;
+    return fail(TG_ERR_HIP, std::string("hipMalloc(twiddles): ") + hipGetErrorString(e));
+  }
+  e = hipMemcpy(p->tw, host.data(), host.size(), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    (void)hipFree(p->tw);
+    delete p;
+    return fail(TG_ERR_HIP, std::string("hipMemcpy(twiddles): ") + hipGetErrorString(e));
+  }
+  *out = p;
+  return TG_OK;
+}
+
+int tg_plan_destroy(tg_plan* p) {
+  if (!p) return TG_OK;
+  if (p->tw) (void)hipFree(p->tw);
+  delete p;
+  return TG_OK;
+}
+
+int tg_plan_query(const tg_plan* p, long long* info) {
+  TG_RET(check_plan(p));
+  if (!info) return fail(TG_ERR_ARG, "info is NULL");
+  info[TG_INFO_N] = p->n;
+  info[TG_INFO_K] = p->K;
+  info[TG_INFO_KZ] = p->KZ;
+  info[TG_INFO_COMPACT_ELEMS] = (long long)p->compact;
+  info[TG_INFO_WS_BYTES] = (long long)p->wsBytes;
+  info[TG_INFO_PRECISION] = p->prec;
+  info[TG_INFO_MHD] = p->mhd;
+  info[7] = 0;
+  return TG_OK;
+}
+
+int tg_forward(tg_plan* p, const void* phys, void* spec, void* ws, void* stream) {
+  TG_RET(check_plan(p));
+  if (!phys || !spec || !ws) return fail(TG_ERR_ARG, "tg_forward: NULL buffer");
+  return with_plan(p, [&](auto ops, auto* rt) {
+    using R = std::remove_pointer_t<decltype(rt)>;
+    return decltype(ops)::forward(p, static_cast<const R*>(phys), static_cast<cpx<R>*>(spec), ws,
+                                  static_cast<hipStream_t>(stream));
+  });
+}
+
+int tg_inverse(tg_plan* p, const void* spec, int op, void* phys, void* ws, void* stream) {
+  TG_RET(check_plan(p));
+  if (!phys || !spec || !ws) return fail(TG_ERR_ARG, "tg_inverse: NULL buffer");
+  if (op != TG_OP_FIELD && op != TG_OP_CURL && op != TG_OP_DIV)
+    return fail(TG_ERR_ARG, "tg_inverse: op must be TG_OP_FIELD, TG_OP_CURL or TG_OP_DIV");
+  return with_plan(p, [&](auto ops, auto* rt) {
+    using R = std::remove_pointer_t<decltype(rt)>;
+    return decltype(ops)::inverse(p, static_cast<const cpx<R>*>(spec), op, static_cast<R*>(phys), ws,
+                                  static_cast<hipStream_t>(stream));
+  });
+}
+
+int tg_rhs(tg_plan* p, const void* u_hat, const void* b_hat, double nu, double eta, double lorentz_sign,
+           void* ru_hat, void* rb_hat, void* ws, void* stream) {
+  TG_RET(check_plan(p));
+  if (!u_hat || !ru_hat || !ws) return fail(TG_ERR_ARG, "tg_rhs: NULL buffer");
+  if (p->mhd && (!b_hat || !rb_hat)) return fail(TG_ERR_ARG, "tg_rhs: MHD plan needs b_hat and rb_hat");
+  return with_plan(p, [&](auto ops, auto* rt) {
+    using R = std::remove_pointer_t<decltype(rt)>;
+    return decltype(ops)::rhs(p, static_cast<const cpx<R>*>(u_hat), static_cast<const cpx<R>*>(b_hat), nu, eta,
+                              lorentz_sign, static_cast<cpx<R>*>(ru_hat), static_cast<cpx<R>*>(rb_hat), ws,
+                              static_cast<hipStream_t>(stream));
+  });
+}
+
+int tg_rk4_steps(tg_plan* p, void* u_hat, void* b_hat, int nsteps, double dt, double nu, double eta,
+                 double lorentz_sign, void* ws, void* stream) {
+  TG_RET(check_plan(p));
+  if (!u_hat || !ws) return fail(TG_ERR_ARG, "tg_rk4_steps: NULL buffer");
+  if (p->mhd && !b_hat) return fail(TG_ERR_ARG, "tg_rk4_steps: MHD plan needs b_hat");
+  if (nsteps < 0) return fail(TG_ERR_ARG, "tg_rk4_steps: nsteps must be >= 0");
+  return with_plan(p, [&](auto ops, auto* rt) {
+    using R = std::remove_pointer_t<decltype(rt)>;
+    return decltype(ops)::rk4(p, static_cast<cpx<R>*>(u_hat), static_cast<cpx<R>*>(b_hat), nsteps, dt, nu, eta,
+                              lorentz_sign, ws, static_cast<hipStream_t>(stream));
+  });
+}
+
+int tg_diagnostics(tg_plan* p, const void* u_hat, const void* b_hat, int with_divmax, double* out, void* ws,
+                   void* stream) {
+  TG_RET(check_plan(p));
+  if (!u_hat || !out || !ws) return fail(TG_ERR_ARG, "tg_diagnostics: NULL buffer");
+  return with_plan(p, [&](auto ops, auto* rt) {
+    using R = std::remove_pointer_t<decltype(rt)>;
+    return decltype(ops)::diagnostics(p, static_cast<const cpx<R>*>(u_hat), static_cast<const cpx<R>*>(b_hat),
+                                      with_divmax, out, ws, static_cast<hipStream_t>(stream));
+  });
+}
+
+int tg_last_error(char* buf, size_t n) {
+  if (!buf || n == 0) return TG_ERR_ARG;
+  snprintf(buf, n, "%s", g_err.c_str());
+  return TG_OK;
+}
+
+}  // extern "C"
