@@ -1,0 +1,185 @@
+"""Benchmark: RK4 steps/s of the hydro Taylor-Green DNS at 256^3 on one MI355X.
+
+Workload: TGV u = sin x cos y cos z, v = -cos x sin y cos z, w = 0 on
+[0, 2 pi]^3, N = 256, nu = 1/1600, dt = 2e-3, classic RK4 with 2/3
+dealiasing and Leray projection; FP64 (the plain drop-in precision) is the
+headline, FP32 is reported beside it.  The initial state is transformed to
+the compact spectrum in HBM before the timed region; the timed region is
+exactly --steps RK4 steps (4 right-hand sides each, every pass on the GPU),
+synchronised on both sides, diagnostics excluded.
+
+Prints ONE JSON line.  --gpus above 1 exits: this path runs on one GPU.
+"""
+import argparse
+import json
+import math
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def tg_initial(N, dtype, device):
+    import torch
+    x = torch.arange(N, dtype=torch.float64, device=device) * (2.0 * math.pi / N)
+    X, Y, Z = torch.meshgrid(x, x, x, indexing="ij")
+    u = torch.stack([torch.sin(X) * torch.cos(Y) * torch.cos(Z),
+                     -torch.cos(X) * torch.sin(Y) * torch.cos(Z),
+                     torch.zeros_like(X)])
+    return u.to(dtype).contiguous()
+
+
+def run_gpu(N, dtype, steps, warmup, dt, nu):
+    import torch
+    from jax_tg_project_amd._lib import Plan
+    plan = Plan(N, dtype=dtype)
+    u0 = tg_initial(N, dtype, plan.device)
+    uh = plan.forward(u0)
+    if warmup:
+        w = uh.clone()
+        plan.rk4(w, nsteps=warmup, dt=dt, nu=nu)
+        del w
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plan.rk4(uh, nsteps=steps, dt=dt, nu=nu)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    diag = plan.diagnostics(uh)
+    # same steps again with the per-pass HIP-event hook on (kernel durations)
+    uh2 = plan.forward(u0)
+    plan.timing(True)
+    plan.rk4(uh2, nsteps=steps, dt=dt, nu=nu)
+    passes = plan.timing_read()
+    plan.timing(False)
+    torch.cuda.synchronize()
+    return {"wall": wall, "diag": diag, "passes": passes, "plan": plan, "u_hat": uh}
+
+
+def cpu_baseline(N, dt, nu, gpu_one_step=None):
+    """One RK4 step of the FP64 oracle (jax/taylor_green_jax.py restated:
+    72 complex 3D FFTs per step) on all host cores given to this process."""
+    import numpy as np
+    from oracle import tg_oracle as o
+    cores = o.cpu_count()
+    o.set_workers(cores)
+    L = 2.0 * math.pi
+    kx, ky, kz = o.make_wavenumbers(N, L)
+    mask = o.make_dealias_mask(N, L)
+    u = o.initial_taylor_green(N)
+    t0 = time.perf_counter()
+    u1 = o.advance_one_step(u, dt, nu, kx, ky, kz, mask)
+    sec = time.perf_counter() - t0
+    out = {"value": 1.0 / sec, "unit": "RK4 steps/s", "cores": cores, "kind": "port",
+           "sample": f"1 RK4 step of the {N}^3 FP64 hydro TGV (72 complex 3D FFTs, scipy.fft "
+                     f"workers={cores}) on {cpu_model()}; {sec:.2f} s"}
+    parity = None
+    if gpu_one_step is not None:
+        d = gpu_one_step - u1
+        parity = float(np.linalg.norm(d.ravel()) / np.linalg.norm(u1.ravel()))
+    return out, parity
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this pseudo-spectral path runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU slab/pencil decomposition is design-only)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from jax_tg_project_amd import roofline
+    N, dt, nu = args.size, 2e-3, 1.0 / 1600.0
+    K, W = args.steps, args.warmup
+
+    r64 = run_gpu(N, torch.float64, K, W, dt, nu)
+    steps_s = K / r64["wall"]
+    pb = roofline.pass_bytes(N, 8)
+    step_b = roofline.step_bytes(N, 8)
+    # dominant kernel = largest total time over the timed steps
+    pas = r64["passes"]
+    dom = max(pas, key=lambda k: pas[k][0])
+    dom_ms_avg = pas[dom][0] / max(1, pas[dom][1])
+    dom_bytes = pb[dom] if dom != "fwd_x" else sum(pb["fwd_x"]) / 4.0
+    achieved = dom_bytes / (dom_ms_avg * 1e-3) / 1e9
+    per_pass = {k: {"ms_avg": v[0] / max(1, v[1]), "launches": v[1],
+                    "GBps": ((pb[k] if k != "fwd_x" else sum(pb["fwd_x"]) / 4.0)
+                             / (v[0] / max(1, v[1]) * 1e-3) / 1e9)} for k, v in pas.items()}
+
+    res = {
+        "metric": "RK4 steps/s + HBM GB/s at 256^3 (hydro TGV, FP64)",
+        "value": steps_s,
+        "unit": "RK4 steps/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": 1e3 * r64["wall"] / K,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (Taylor-Green initial condition, generated on the GPU)",
+        "config": {"workload": f"hydro TGV N={N}^3 FP64, nu=1/1600, dt=2e-3, classic RK4, 2/3 dealiasing, "
+                               f"Leray projection, {K} steps", "N": N, "nu": nu, "dt": dt},
+        "hbm_GBps_algorithmic_step": step_b / (r64["wall"] / K) / 1e9,
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_launch": dom_bytes, "ms_per_launch": dom_ms_avg},
+        "passes_fp64": per_pass,
+        "energy_fp64_final": r64["diag"]["kinetic_energy"],
+        "enstrophy_fp64_final": r64["diag"]["enstrophy"],
+        "div_max_fp64_final": r64["diag"]["div_u_max"],
+    }
+    e64 = r64["diag"]["kinetic_energy"]
+    del r64
+    torch.cuda.empty_cache()
+
+    if not args.no_fp32:
+        r32 = run_gpu(N, torch.float32, K, W, dt, nu)
+        res["fp32_steps_per_s"] = K / r32["wall"]
+        res["fp32_over_fp64"] = res["fp32_steps_per_s"] / steps_s
+        res["fp32_energy_rel_err_vs_fp64"] = abs(r32["diag"]["kinetic_energy"] - e64) / e64
+        res["passes_fp32"] = {k: {"ms_avg": v[0] / max(1, v[1]), "launches": v[1]}
+                              for k, v in r32["passes"].items()}
+        del r32
+        torch.cuda.empty_cache()
+
+    if not args.no_cpu_baseline:
+        from jax_tg_project_amd._lib import Plan
+        plan = Plan(N, dtype=torch.float64)
+        uh = plan.forward(tg_initial(N, torch.float64, plan.device))
+        plan.rk4(uh, nsteps=1, dt=dt, nu=nu)
+        one = plan.inverse(uh).cpu().numpy()
+        del plan, uh
+        torch.cuda.empty_cache()
+        cb, parity = cpu_baseline(N, dt, nu, one)
+        res["cpu_baseline"] = cb
+        res["parity_1step_rel_l2_vs_oracle"] = parity
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

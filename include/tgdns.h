@@ -118,6 +118,20 @@ int tg_rk4_steps(tg_plan* plan, void* u_hat, void* b_hat, int nsteps, double dt,
 int tg_diagnostics(tg_plan* plan, const void* u_hat, const void* b_hat, int with_divmax,
                    double* out, void* ws, void* stream);
 
+/* Per-pass timing hook (no counterpart in the reference: measurement only).
+ * While enabled, every pass launch of tg_rhs / tg_rk4_steps is bracketed by
+ * two HIP events recorded on the launch stream; tg_timing_read waits for the
+ * last one and returns, per pass id, the summed elapsed milliseconds and the
+ * number of launches since the previous read (or enable). */
+#define TG_PASS_INV_X 0 /* inverse x + derived fields (u, curl u [, b, curl b]) */
+#define TG_PASS_INV_Y 1 /* inverse y                                            */
+#define TG_PASS_Z 2     /* z: C2R, nonlinear product, R2C                       */
+#define TG_PASS_FWD_Y 3 /* forward y                                            */
+#define TG_PASS_FWD_X 4 /* forward x + projection + viscous + RK4 stage         */
+#define TG_NPASS 5
+int tg_timing_enable(tg_plan* plan, int on);
+int tg_timing_read(tg_plan* plan, double* ms, long long* count);
+
 /* Copy the calling thread's last error message into buf (NUL-terminated). */
 int tg_last_error(char* buf, size_t n);
 

@@ -36,8 +36,12 @@ _SIGS = {
                       ctypes.c_double, ctypes.c_double, _VP, _VP], ctypes.c_int),
     "tg_diagnostics": ([_VP, _VP, _VP, ctypes.c_int, ctypes.POINTER(ctypes.c_double), _VP, _VP],
                        ctypes.c_int),
+    "tg_timing_enable": ([_VP, ctypes.c_int], ctypes.c_int),
+    "tg_timing_read": ([_VP, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_longlong)],
+                       ctypes.c_int),
     "tg_last_error": ([ctypes.c_char_p, ctypes.c_size_t], ctypes.c_int),
 }
+PASS_NAMES = ("inv_x", "inv_y", "z", "fwd_y", "fwd_x")
 EXPORTED = tuple(_SIGS)
 
 
@@ -197,6 +201,17 @@ class Plan:
             _check(self._lib.tg_rk4_steps(self._h, _ptr(u_hat), _ptr(b_hat), int(nsteps), float(dt),
                                           float(nu), float(eta), float(lorentz_sign), _ptr(self.ws),
                                           _stream()))
+
+    def timing(self, on):
+        """Enable/disable the per-pass HIP-event timing hook."""
+        _check(self._lib.tg_timing_enable(self._h, int(bool(on))))
+
+    def timing_read(self):
+        """{pass name: (total ms, launches)} since the last read; waits for the GPU."""
+        ms = (ctypes.c_double * len(PASS_NAMES))()
+        cnt = (ctypes.c_longlong * len(PASS_NAMES))()
+        _check(self._lib.tg_timing_read(self._h, ms, cnt))
+        return {n: (float(ms[i]), int(cnt[i])) for i, n in enumerate(PASS_NAMES)}
 
     def diagnostics(self, u_hat, b_hat=None, divmax=True):
         """Dict of FP64 diagnostics of the compact state (synchronises)."""

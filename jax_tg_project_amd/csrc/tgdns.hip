@@ -52,6 +52,11 @@ struct tg_plan {
   // workspace layout (byte offsets)
   size_t offA, offB, offU1, offAcc, offPart, offMax, offOut, wsBytes;
   int nDiagBlk, nMax;
+  // optional per-pass timing: pairs of events around each pass launch
+  bool timing = false;
+  std::vector<hipEvent_t> evpool;
+  std::vector<int> evpass;   // pass id of each recorded pair
+  size_t evused = 0;         // pairs recorded since the last read
 };
 
 namespace {
@@ -156,6 +161,38 @@ struct Launch {
     if (r_) return r_;       \
   } while (0)
 
+// Pass ids for the timing hook (TG_PASS_* in include/tgdns.h).
+int timed_begin(const tg_plan* cp, int pass, hipStream_t s) {
+  tg_plan* p = const_cast<tg_plan*>(cp);
+  if (!p->timing) return TG_OK;
+  if (2 * (p->evused + 1) > p->evpool.size()) {
+    for (int i = 0; i < 64; ++i) {
+      hipEvent_t e;
+      TG_HIP(hipEventCreate(&e));
+      p->evpool.push_back(e);
+    }
+  }
+  p->evpass.resize(p->evused + 1);
+  p->evpass[p->evused] = pass;
+  TG_HIP(hipEventRecord(p->evpool[2 * p->evused], s));
+  return TG_OK;
+}
+
+int timed_end(const tg_plan* cp, hipStream_t s) {
+  tg_plan* p = const_cast<tg_plan*>(cp);
+  if (!p->timing) return TG_OK;
+  TG_HIP(hipEventRecord(p->evpool[2 * p->evused + 1], s));
+  ++p->evused;
+  return TG_OK;
+}
+
+#define TG_TIMED(pass, ...)                  \
+  do {                                       \
+    TG_RET(timed_begin(p, (pass), s));       \
+    TG_RET(__VA_ARGS__);                     \
+    TG_RET(timed_end(p, s));                 \
+  } while (0)
+
 // One right-hand side: stage input (us, bs) -> z-pass input in A/B -> the
 // forward-x epilogue described by `mode` / `a`.
 template <typename R, int N>
@@ -165,11 +202,11 @@ int rhs_passes(const tg_plan* p, const Ptrs<R>& w, const cpx<R>* us, const cpx<R
   const cpx<R>* tw = static_cast<const cpx<R>*>(p->tw);
   const bool mhd = p->mhd != 0;
   const int fi = mhd ? 12 : 6, fo = mhd ? 6 : 3;
-  TG_RET(Lc::inv_x(mhd ? XM_MHD : XM_HYDRO, us, bs, w.A, tw, R(p->ks), s));
-  TG_RET(Lc::y(+1, fi, w.A, w.B, tw, s));
-  TG_RET(Lc::z(mhd ? ZM_MHD : ZM_HYDRO, w.B, w.A, nullptr, nullptr, tw, lor, s));
-  TG_RET(Lc::y(-1, fo, w.A, w.B, tw, s));
-  TG_RET(Lc::fwd_x(fmode, w.B, a, tw, s));
+  TG_TIMED(TG_PASS_INV_X, Lc::inv_x(mhd ? XM_MHD : XM_HYDRO, us, bs, w.A, tw, R(p->ks), s));
+  TG_TIMED(TG_PASS_INV_Y, Lc::y(+1, fi, w.A, w.B, tw, s));
+  TG_TIMED(TG_PASS_Z, Lc::z(mhd ? ZM_MHD : ZM_HYDRO, w.B, w.A, nullptr, nullptr, tw, lor, s));
+  TG_TIMED(TG_PASS_FWD_Y, Lc::y(-1, fo, w.A, w.B, tw, s));
+  TG_TIMED(TG_PASS_FWD_X, Lc::fwd_x(fmode, w.B, a, tw, s));
   return TG_OK;
 }
 
@@ -396,6 +433,7 @@ int tg_plan_create(int n, int precision_bits, int mhd, double domain_length, tg_
 
 int tg_plan_destroy(tg_plan* p) {
   if (!p) return TG_OK;
+  for (hipEvent_t e : p->evpool) (void)hipEventDestroy(e);
   if (p->tw) (void)hipFree(p->tw);
   delete p;
   return TG_OK;
@@ -472,6 +510,28 @@ int tg_diagnostics(tg_plan* p, const void* u_hat, const void* b_hat, int with_di
     return decltype(ops)::diagnostics(p, static_cast<const cpx<R>*>(u_hat), static_cast<const cpx<R>*>(b_hat),
                                       with_divmax, out, ws, static_cast<hipStream_t>(stream));
   });
+}
+
+int tg_timing_enable(tg_plan* p, int on) {
+  TG_RET(check_plan(p));
+  p->timing = on != 0;
+  p->evused = 0;
+  return TG_OK;
+}
+
+int tg_timing_read(tg_plan* p, double* ms, long long* count) {
+  TG_RET(check_plan(p));
+  if (!ms || !count) return fail(TG_ERR_ARG, "tg_timing_read: NULL output");
+  for (int i = 0; i < TG_NPASS; ++i) { ms[i] = 0.0; count[i] = 0; }
+  if (p->evused) TG_HIP(hipEventSynchronize(p->evpool[2 * p->evused - 1]));
+  for (size_t i = 0; i < p->evused; ++i) {
+    float t = 0.f;
+    TG_HIP(hipEventElapsedTime(&t, p->evpool[2 * i], p->evpool[2 * i + 1]));
+    ms[p->evpass[i]] += t;
+    count[p->evpass[i]] += 1;
+  }
+  p->evused = 0;
+  return TG_OK;
 }
 
 int tg_last_error(char* buf, size_t n) {
