@@ -46,10 +46,12 @@ def tg_initial(N, dtype, device):
     return u.to(dtype).contiguous()
 
 
-def run_gpu(N, dtype, steps, warmup, dt, nu):
+def run_gpu(N, dtype, steps, warmup, dt, nu, tune=True):
     import torch
     from jax_tg_project_amd._lib import Plan
+    from jax_tg_project_amd import autotune
     plan = Plan(N, dtype=dtype)
+    tuned = autotune.tune(plan, use_cache=False)[0] if tune else plan.get_variants()
     u0 = tg_initial(N, dtype, plan.device)
     uh = plan.forward(u0)
     if warmup:
@@ -69,7 +71,7 @@ def run_gpu(N, dtype, steps, warmup, dt, nu):
     passes = plan.timing_read()
     plan.timing(False)
     torch.cuda.synchronize()
-    return {"wall": wall, "diag": diag, "passes": passes, "plan": plan, "u_hat": uh}
+    return {"wall": wall, "diag": diag, "passes": passes, "plan": plan, "u_hat": uh, "variants": tuned}
 
 
 def cpu_baseline(N, dt, nu, gpu_one_step=None):
@@ -104,6 +106,7 @@ def main():
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-tune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this pseudo-spectral path runs on one GPU; --gpus {args.gpus} is not supported "
@@ -115,7 +118,7 @@ def main():
     N, dt, nu = args.size, 2e-3, 1.0 / 1600.0
     K, W = args.steps, args.warmup
 
-    r64 = run_gpu(N, torch.float64, K, W, dt, nu)
+    r64 = run_gpu(N, torch.float64, K, W, dt, nu, tune=not args.no_tune)
     steps_s = K / r64["wall"]
     pb = roofline.pass_bytes(N, 8)
     step_b = roofline.step_bytes(N, 8)
@@ -149,6 +152,7 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "bytes_per_launch": dom_bytes, "ms_per_launch": dom_ms_avg},
         "passes_fp64": per_pass,
+        "tile_variants_fp64": r64["variants"],
         "energy_fp64_final": r64["diag"]["kinetic_energy"],
         "enstrophy_fp64_final": r64["diag"]["enstrophy"],
         "div_max_fp64_final": r64["diag"]["div_u_max"],
@@ -158,12 +162,13 @@ def main():
     torch.cuda.empty_cache()
 
     if not args.no_fp32:
-        r32 = run_gpu(N, torch.float32, K, W, dt, nu)
+        r32 = run_gpu(N, torch.float32, K, W, dt, nu, tune=not args.no_tune)
         res["fp32_steps_per_s"] = K / r32["wall"]
         res["fp32_over_fp64"] = res["fp32_steps_per_s"] / steps_s
         res["fp32_energy_rel_err_vs_fp64"] = abs(r32["diag"]["kinetic_energy"] - e64) / e64
         res["passes_fp32"] = {k: {"ms_avg": v[0] / max(1, v[1]), "launches": v[1]}
                               for k, v in r32["passes"].items()}
+        res["tile_variants_fp32"] = r32["variants"]
         del r32
         torch.cuda.empty_cache()
 

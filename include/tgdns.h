@@ -130,6 +130,15 @@ int tg_diagnostics(tg_plan* plan, const void* u_hat, const void* b_hat, int with
 #define TG_PASS_FWD_X 4 /* forward x + projection + viscous + RK4 stage         */
 #define TG_NPASS 5
 int tg_timing_enable(tg_plan* plan, int on);
+
+/* Tile variants (no counterpart in the reference: tuning).  Each pass of
+ * tg_rhs / tg_rk4_steps has `count` compiled tile shapes (3 for N >= 128,
+ * else 1).  If `set` is non-NULL, set[TG_NPASS] selects the variant per
+ * pass (TG_PASS_* order); if `get` is non-NULL it receives the current
+ * selection.  Results do not depend on the variant beyond round-off order of
+ * the same butterflies; the autotuner (jax_tg_project_amd/autotune.py)
+ * times them and stores the fastest per (device, N, precision, equations). */
+int tg_plan_variants(tg_plan* plan, const int* set, int* get, int* count);
 int tg_timing_read(tg_plan* plan, double* ms, long long* count);
 
 /* Copy the calling thread's last error message into buf (NUL-terminated). */

@@ -12,7 +12,7 @@ import os
 import torch
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libtgdns.so")
+LIB_PATH = os.environ.get("TGDNS_LIB") or os.path.join(_HERE, "libtgdns.so")
 
 TG_INFO_N, TG_INFO_K, TG_INFO_KZ, TG_INFO_COMPACT_ELEMS, TG_INFO_WS_BYTES = 0, 1, 2, 3, 4
 TG_INFO_PRECISION, TG_INFO_MHD, TG_INFO_COUNT = 5, 6, 8
@@ -36,6 +36,8 @@ _SIGS = {
                       ctypes.c_double, ctypes.c_double, _VP, _VP], ctypes.c_int),
     "tg_diagnostics": ([_VP, _VP, _VP, ctypes.c_int, ctypes.POINTER(ctypes.c_double), _VP, _VP],
                        ctypes.c_int),
+    "tg_plan_variants": ([_VP, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                          ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
     "tg_timing_enable": ([_VP, ctypes.c_int], ctypes.c_int),
     "tg_timing_read": ([_VP, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_longlong)],
                        ctypes.c_int),
@@ -201,6 +203,20 @@ class Plan:
             _check(self._lib.tg_rk4_steps(self._h, _ptr(u_hat), _ptr(b_hat), int(nsteps), float(dt),
                                           float(nu), float(eta), float(lorentz_sign), _ptr(self.ws),
                                           _stream()))
+
+    def variant_count(self):
+        cnt = ctypes.c_int()
+        _check(self._lib.tg_plan_variants(self._h, None, None, ctypes.byref(cnt)))
+        return int(cnt.value)
+
+    def get_variants(self):
+        out = (ctypes.c_int * len(PASS_NAMES))()
+        _check(self._lib.tg_plan_variants(self._h, None, out, None))
+        return [int(v) for v in out]
+
+    def set_variants(self, variants):
+        arr = (ctypes.c_int * len(PASS_NAMES))(*[int(v) for v in variants])
+        _check(self._lib.tg_plan_variants(self._h, arr, None, None))
 
     def timing(self, on):
         """Enable/disable the per-pass HIP-event timing hook."""

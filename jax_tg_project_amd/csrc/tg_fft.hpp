@@ -231,6 +231,93 @@ __device__ __forceinline__ void fft_apply(cpx<R>* buf, int LP, int l, int t, boo
   run_stages<R, N, E, DIR, 1, Rs...>(buf, LP, l, t, active, tw);
 }
 
+// ---------------------------------------------------------------- global I/O
+// Stage with optional direct I/O.  SRC: 0 = LDS, 1 = functor (no LDS read, so
+// no barrier is needed before this stage writes LDS), 2 = functor that reads
+// LDS itself (barrier before writing).  DST_GLOBAL: results go to the store
+// functor instead of LDS (then no trailing barrier).
+template <typename R, int N, int E, int DIR, int P, int RAD, int SRC, bool DST_GLOBAL, class Ld, class St>
+__device__ __forceinline__ void io_stage(cpx<R>* buf, int LP, int l, int t, bool active,
+                                         const cpx<R>* __restrict__ tw, const Ld& ld, const St& st) {
+  static_assert(E % RAD == 0, "radix must divide points per thread");
+  constexpr int NB = E / RAD;
+  constexpr int SIN = N / RAD;
+  cpx<R> v[NB][RAD];
+  if (active) {
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+      const int b = t * NB + j;
+#pragma unroll
+      for (int m = 0; m < RAD; ++m) {
+        const int n = b + m * SIN;
+        if constexpr (SRC == 0) v[j][m] = buf[n * LP + l];
+        else v[j][m] = ld(n, l);
+      }
+    }
+  }
+  if constexpr (SRC != 1 && !DST_GLOBAL) __syncthreads();
+  if (active) {
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+      const int b = t * NB + j;
+      const int k = b % P;
+      if constexpr (P > 1) {
+#pragma unroll
+        for (int m = 1; m < RAD; ++m) {
+          cpx<R> w = tw[m * k * (N / (P * RAD))];
+          if constexpr (DIR > 0) w.y = -w.y;
+          v[j][m] = cmul(v[j][m], w);
+        }
+      }
+      dft<RAD, DIR>(v[j]);
+      const int base = (b - k) * RAD + k;
+#pragma unroll
+      for (int m = 0; m < RAD; ++m) {
+        if constexpr (DST_GLOBAL) st(base + m * P, l, v[j][m]);
+        else buf[(base + m * P) * LP + l] = v[j][m];
+      }
+    }
+  }
+  if constexpr (!DST_GLOBAL) __syncthreads();
+}
+
+template <typename R, int N, int E, int DIR, int SRC, bool DST_GLOBAL, int P, class Ld, class St, int R0,
+          int... Rest>
+__device__ __forceinline__ void io_stages(cpx<R>* buf, int LP, int l, int t, bool active,
+                                          const cpx<R>* __restrict__ tw, const Ld& ld, const St& st) {
+  constexpr bool last = sizeof...(Rest) == 0;
+  io_stage<R, N, E, DIR, P, R0, SRC, last && DST_GLOBAL>(buf, LP, l, t, active, tw, ld, st);
+  if constexpr (!last)
+    io_stages<R, N, E, DIR, 0, DST_GLOBAL, P * R0, Ld, St, Rest...>(buf, LP, l, t, active, tw, ld, st);
+}
+
+template <typename R, int N, int E, int DIR, int SRC, bool DST_GLOBAL, class Ld, class St, int... Rs>
+__device__ __forceinline__ void io_apply(cpx<R>* buf, int LP, int l, int t, bool active,
+                                         const cpx<R>* __restrict__ tw, const Ld& ld, const St& st,
+                                         Radices<Rs...>) {
+  io_stages<R, N, E, DIR, SRC, DST_GLOBAL, 1, Ld, St, Rs...>(buf, LP, l, t, active, tw, ld, st);
+}
+
+// Transform `nlines` lines whose first stage reads through `ld(n, l)` and
+// (if DST_GLOBAL) whose last stage writes through `st(n, l, v)` in natural
+// order; intermediate stages exchange through LDS (buf[n*LP + l]).  The
+// caller guarantees the LDS buffer is free on entry (barrier after any
+// earlier use).  With DST_GLOBAL the call ends without a barrier.
+struct NoIO {
+  template <typename... A>
+  __device__ void operator()(A...) const {}
+};
+
+template <typename R, int N, int E, int DIR, int SRC, bool DST_GLOBAL, class Ld, class St>
+__device__ __forceinline__ void fft_io(cpx<R>* buf, int LP, int nlines, const cpx<R>* __restrict__ tw,
+                                       const Ld& ld, const St& st) {
+  const int tid = threadIdx.x;
+  const int l = tid % nlines;
+  const int t = tid / nlines;
+  const bool active = t < N / E;
+  io_apply<R, N, E, DIR, SRC, DST_GLOBAL>(buf, LP, l, t, active, tw, ld, st, typename PlanFor<N, E>::type{});
+}
+
 // Transform `nlines` lines of length N held in LDS (buf[n*LP + l]).  The
 // calling block must have at least nlines * N / E threads; the rest idle but
 // take part in the barriers.  Entry and exit are barrier-separated.

@@ -28,25 +28,39 @@ struct Geo {
   __device__ static constexpr int kint(int i) { return i <= M ? i : i - K; }
 };
 
-// Launch shapes.  X/Y passes: L columns of kz per block, E points per thread.
+// Launch shapes.  Grids of N >= 128 compile NVAR tile variants per pass;
+// the plan picks one per pass at run time (see the autotuner); smaller grids
+// compile variant 0 only.
+constexpr int NVAR = 3;
 template <int N>
+constexpr int num_variants() { return N >= 128 ? NVAR : 1; }
+
+template <int N>
+constexpr int base_e() { return (N == 48 || N == 96) ? 12 : (N >= 128 ? 16 : (N == 4 ? 4 : 8)); }
+
+// X/Y passes: L columns of kz per block, E points per thread.
+//   V0: 16 columns x 16 points   V1: 8 columns x 8 points   V2: 32 columns x 16 points
+template <typename R, int N, int V>
 struct CfgXY {
-  static constexpr int E = (N == 48 || N == 96) ? 12 : (N >= 128 ? 16 : (N == 4 ? 4 : 8));
-  static constexpr int L = 16;
+  static constexpr int E = (V == 1) ? 8 : base_e<N>();
+  static constexpr int L = V == 1 ? 8 : (V == 2 ? 32 : 16);
   static constexpr int NT = L * N / E;
 };
-// Z pass: E points per thread for the inverse, E/2 for the forward (which
-// has half as many sequences).  NB = 3 x lines per block for hydro.
-template <int N>
+// Z pass: NB = 3 x (hydro lines per block); E points per thread for the
+// inverse, E/2 for the forward (half as many sequences).
+//   V0: NB = 12 (N >= 256) or 24    V1: twice V0    V2: half of V0
+template <typename R, int N, int V>
 struct CfgZ {
-  static constexpr int NB = (N >= 256 || N == 4) ? 12 : 24;
+  static constexpr int NB0 = N >= 256 ? 12 : (N == 4 ? 12 : 24);
+  static constexpr int NB = V == 1 ? 2 * NB0 : (V == 2 ? NB0 / 2 : NB0);
   static constexpr int E = (N == 48 || N == 96) ? 24 : (N >= 128 ? 16 : (N == 4 ? 4 : 8));
 };
-// Forward-x stage pass: 24 lines per block (3 fields x 8 kz or 6 x 4).
-template <int N>
+// Forward-x stage pass: LINES lines per block (3 fields x LINES/3 kz, or 6 x LINES/6).
+//   V0: 24 lines, E = 16   V1: 24 lines, E = 8   V2: 48 lines (FP32) / 12 lines (FP64), E = 16
+template <typename R, int N, int V>
 struct CfgXF {
-  static constexpr int E = CfgXY<N>::E;
-  static constexpr int LINES = 24;
+  static constexpr int E = (V == 1) ? 8 : base_e<N>();
+  static constexpr int LINES = V == 2 ? (sizeof(R) == 4 ? 48 : 12) : 24;
   static constexpr int NT = LINES * N / E;
 };
 
@@ -85,13 +99,14 @@ __device__ __forceinline__ cpx<R> xfield(int f, const cpx<R>* __restrict__ su,
   return cmuli(r);
 }
 
-template <typename R, int N, int MODE>
-__global__ __launch_bounds__(CfgXY<N>::NT) void k_inv_x(const cpx<R>* __restrict__ su,
+template <typename R, int N, int MODE, int V>
+__global__ __launch_bounds__((CfgXY<R, N, V>::NT)) void k_inv_x(const cpx<R>* __restrict__ su,
                                                         const cpx<R>* __restrict__ sb,
                                                         cpx<R>* __restrict__ out,
                                                         const cpx<R>* __restrict__ tw, R ks) {
   using G = Geo<N>;
-  constexpr int L = CfgXY<N>::L, E = CfgXY<N>::E, NT = CfgXY<N>::NT;
+  using C = CfgXY<R, N, V>;
+  constexpr int L = C::L, E = C::E;
   constexpr int F = xmode_fields(MODE);
   __shared__ cpx<R> buf[N * L];
   const int iz0 = blockIdx.x * L, iy = blockIdx.y;
@@ -100,41 +115,41 @@ __global__ __launch_bounds__(CfgXY<N>::NT) void k_inv_x(const cpx<R>* __restrict
   const R scale = R(1) / (R(N) * R(N) * R(N));
   const R ky = ks * R(G::kint(iy));
   for (int f = 0; f < F; ++f) {
-    for (int idx = tid; idx < N * L; idx += NT) {
-      const int n = idx / L, ll = idx % L, iz = iz0 + ll;
+    if (f) __syncthreads();  // the previous field's last LDS reads are done
+    auto ld = [&](int n, int ll) -> cpx<R> {
+      const int iz = iz0 + ll;
       cpx<R> v = {R(0), R(0)};
       if (iz < G::KZ && G::kept(n)) {
         const int ix = G::compact_of(n);
         const size_t pt = (size_t(ix) * G::K + iy) * G::KZ + iz;
         v = cscale(xfield<R, N, MODE>(f, su, sb, pt, cs, ks * R(G::kint(ix)), ky, ks * R(iz)), scale);
       }
-      buf[n * L + ll] = v;
-    }
-    __syncthreads();
-    fft_lines<R, N, E, +1>(buf, L, L, tw);
-    for (int idx = tid; idx < N * L; idx += NT) {
-      const int n = idx / L, ll = idx % L, iz = iz0 + ll;
-      if (iz < G::KZ) out[((size_t(f) * N + n) * G::K + iy) * G::KZ + iz] = buf[n * L + ll];
-    }
-    __syncthreads();
+      return v;
+    };
+    auto st = [&](int n, int ll, cpx<R> v) {
+      const int iz = iz0 + ll;
+      if (iz < G::KZ) out[((size_t(f) * N + n) * G::K + iy) * G::KZ + iz] = v;
+    };
+    fft_io<R, N, E, +1, 1, true>(buf, L, L, tw, ld, st);
   }
 }
 
 // ----------------------------------------------------------------- pass 2/4
 // Transform along y for F fields.  DIR=+1: A[f][x][iy][iz] (K rows, zero
 // padded) -> B[f][x][y][iz].  DIR=-1: B -> A keeping the K masked rows.
-template <typename R, int N, int DIR>
-__global__ __launch_bounds__(CfgXY<N>::NT) void k_y(const cpx<R>* __restrict__ in,
+template <typename R, int N, int DIR, int V>
+__global__ __launch_bounds__((CfgXY<R, N, V>::NT)) void k_y(const cpx<R>* __restrict__ in,
                                                     cpx<R>* __restrict__ out,
                                                     const cpx<R>* __restrict__ tw) {
   using G = Geo<N>;
-  constexpr int L = CfgXY<N>::L, E = CfgXY<N>::E, NT = CfgXY<N>::NT;
+  using C = CfgXY<R, N, V>;
+  constexpr int L = C::L, E = C::E;
   __shared__ cpx<R> buf[N * L];
   const int iz0 = blockIdx.x * L, x = blockIdx.y, f = blockIdx.z;
   const int tid = threadIdx.x;
   const size_t plane = size_t(f) * N + x;
-  for (int idx = tid; idx < N * L; idx += NT) {
-    const int n = idx / L, ll = idx % L, iz = iz0 + ll;
+  auto ld = [&](int n, int ll) -> cpx<R> {
+    const int iz = iz0 + ll;
     cpx<R> v = {R(0), R(0)};
     if (iz < G::KZ) {
       if constexpr (DIR > 0) {
@@ -143,19 +158,18 @@ __global__ __launch_bounds__(CfgXY<N>::NT) void k_y(const cpx<R>* __restrict__ i
         v = in[(plane * N + n) * G::KZ + iz];
       }
     }
-    buf[n * L + ll] = v;
-  }
-  __syncthreads();
-  fft_lines<R, N, E, DIR>(buf, L, L, tw);
-  for (int idx = tid; idx < N * L; idx += NT) {
-    const int n = idx / L, ll = idx % L, iz = iz0 + ll;
-    if (iz >= G::KZ) continue;
+    return v;
+  };
+  auto st = [&](int n, int ll, cpx<R> v) {
+    const int iz = iz0 + ll;
+    if (iz >= G::KZ) return;
     if constexpr (DIR > 0) {
-      out[(plane * N + n) * G::KZ + iz] = buf[n * L + ll];
+      out[(plane * N + n) * G::KZ + iz] = v;
     } else {
-      if (G::kept(n)) out[(plane * G::K + G::compact_of(n)) * G::KZ + iz] = buf[n * L + ll];
+      if (G::kept(n)) out[(plane * G::K + G::compact_of(n)) * G::KZ + iz] = v;
     }
-  }
+  };
+  fft_io<R, N, E, DIR, 1, true>(buf, L, L, tw, ld, st);
 }
 
 // ----------------------------------------------------------------- pass 3
@@ -173,11 +187,11 @@ template <> struct ZShape<ZM_PHYS_IN3> { static constexpr int SPL = 3, FIN = 0, 
 template <> struct ZShape<ZM_PHYS_OUT1> { static constexpr int SPL = 1, FIN = 1, FOUT = 0; };
 
 // Per-mode block shape: LZ lines, NSEQ = LZ * SPL inverse-side sequences.
-template <int N, int MODE>
+template <typename R, int N, int MODE, int V>
 struct CfgZM {
-  static constexpr int LZ = (MODE == ZM_MHD) ? CfgZ<N>::NB / 6 : CfgZ<N>::NB / 3;
+  static constexpr int LZ = (MODE == ZM_MHD) ? CfgZ<R, N, V>::NB / 6 : CfgZ<R, N, V>::NB / 3;
   static constexpr int NSEQ = LZ * ZShape<MODE>::SPL;
-  static constexpr int E = CfgZ<N>::E;
+  static constexpr int E = CfgZ<R, N, V>::E;
   static constexpr int NT = NSEQ * N / E;
   static constexpr int LP = NSEQ | 1;  // odd pitch: conflict-free column access
 };
@@ -189,14 +203,14 @@ __device__ __forceinline__ void cross3(const R* a, const R* b, R* o) {
   o[2] = a[0] * b[1] - a[1] * b[0];
 }
 
-template <typename R, int N, int MODE>
-__global__ __launch_bounds__((CfgZM<N, MODE>::NT)) void k_z(const cpx<R>* __restrict__ in,
+template <typename R, int N, int MODE, int V>
+__global__ __launch_bounds__((CfgZM<R, N, MODE, V>::NT)) void k_z(const cpx<R>* __restrict__ in,
                                                    cpx<R>* __restrict__ out,
                                                    const R* __restrict__ phys_in,
                                                    R* __restrict__ phys_out,
                                                    const cpx<R>* __restrict__ tw, R lorentz) {
   using G = Geo<N>;
-  using Z = CfgZM<N, MODE>;
+  using Z = CfgZM<R, N, MODE, V>;
   using S = ZShape<MODE>;
   constexpr int NSEQ = Z::NSEQ, LP = Z::LP, NT = Z::NT, E = Z::E;
   constexpr int LZ = Z::LZ;  // lines per block
@@ -209,9 +223,9 @@ __global__ __launch_bounds__((CfgZM<N, MODE>::NT)) void k_z(const cpx<R>* __rest
   const size_t line0 = size_t(x) * N + y0;  // (x, y0) line index
 
   if constexpr (MODE != ZM_PHYS_IN3) {
-    // ---- fill: sequence s = line*SPL + p packs fields (fa, fb)
-    for (int idx = tid; idx < NSEQ * N; idx += NT) {
-      const int s = idx / N, k = idx % N;
+    // ---- inverse: sequence s = line*SPL + p packs fields (fa, fb) as a + i b;
+    // the first FFT stage reads the Hermitian-extended coefficients directly.
+    auto ld = [&](int k, int s) -> cpx<R> {
       const int line = s / S::SPL, p = s % S::SPL;
       int fa, fb;
       if constexpr (MODE == ZM_HYDRO) { fa = p; fb = p + 3; }
@@ -228,10 +242,9 @@ __global__ __launch_bounds__((CfgZM<N, MODE>::NT)) void k_z(const cpx<R>* __rest
         if (k == 0) { a.y = R(0); b.y = R(0); }
         if (neg) { a.y = -a.y; b.y = -b.y; }
       }
-      buf[k * LP + s] = {a.x - b.y, a.y + b.x};  // a + i b
-    }
-    __syncthreads();
-    fft_lines<R, N, E, +1>(buf, LP, NSEQ, tw);
+      return {a.x - b.y, a.y + b.x};
+    };
+    fft_io<R, N, E, +1, 1, false>(buf, LP, NSEQ, tw, ld, NoIO{});
   }
 
   if constexpr (MODE == ZM_PHYS_OUT3) {
@@ -407,27 +420,25 @@ __device__ __forceinline__ void rk4_update(int stage, R dt, size_t pt, size_t cs
   }
 }
 
-template <typename R, int N, int MODE>
-__global__ __launch_bounds__(CfgXF<N>::NT) void k_fwd_x(const cpx<R>* __restrict__ in,
+template <typename R, int N, int MODE, int V>
+__global__ __launch_bounds__((CfgXF<R, N, V>::NT)) void k_fwd_x(const cpx<R>* __restrict__ in,
                                                         StageArgs<R> a,
                                                         const cpx<R>* __restrict__ tw) {
   using G = Geo<N>;
   constexpr int FF = (MODE == FM_MHD_STAGE || MODE == FM_MHD_RHS) ? 6 : 3;
-  constexpr int LINES = CfgXF<N>::LINES, NT = CfgXF<N>::NT, E = CfgXF<N>::E;
+  constexpr int LINES = CfgXF<R, N, V>::LINES, NT = CfgXF<R, N, V>::NT, E = CfgXF<R, N, V>::E;
   constexpr int L = LINES / FF;
   constexpr int LP = LINES;
   __shared__ cpx<R> buf[N * LP];
   const int iz0 = blockIdx.x * L, iy = blockIdx.y;
   const int tid = threadIdx.x;
-  for (int idx = tid; idx < FF * N * L; idx += NT) {
-    const int ll = idx % L, n = (idx / L) % N, f = idx / (L * N);
-    const int iz = iz0 + ll;
+  auto ld = [&](int n, int ln) -> cpx<R> {
+    const int f = ln / L, iz = iz0 + ln % L;
     cpx<R> v = {R(0), R(0)};
     if (iz < G::KZ) v = in[((size_t(f) * N + n) * G::K + iy) * G::KZ + iz];
-    buf[n * LP + f * L + ll] = v;
-  }
-  __syncthreads();
-  fft_lines<R, N, E, -1>(buf, LP, LINES, tw);
+    return v;
+  };
+  fft_io<R, N, E, -1, 1, false>(buf, LP, LINES, tw, ld, NoIO{});
   const size_t cs = size_t(G::K) * G::K * G::KZ;
   const R ky = a.ks * R(G::kint(iy));
   for (int idx = tid; idx < G::K * L; idx += NT) {

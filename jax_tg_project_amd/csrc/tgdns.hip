@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/tgdns.h"
@@ -52,6 +53,7 @@ struct tg_plan {
   // workspace layout (byte offsets)
   size_t offA, offB, offU1, offAcc, offPart, offMax, offOut, wsBytes;
   int nDiagBlk, nMax;
+  int var[TG_NPASS] = {0, 0, 0, 0, 0};  // tile variant per pass
   // optional per-pass timing: pairs of events around each pass launch
   bool timing = false;
   std::vector<hipEvent_t> evpool;
@@ -91,62 +93,99 @@ Ptrs<R> carve(const tg_plan* p, void* ws) {
 inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 
 // ---------------------------------------------------------------- launchers
+template <int V>
+using VC = std::integral_constant<int, V>;
+
+// Call f(VC<v>) for the run-time tile variant v (0 when N has one variant).
+template <int N, class F>
+int with_variant(int v, F&& f) {
+  if constexpr (num_variants<N>() == 1) {
+    return f(VC<0>{});
+  } else {
+    switch (v) {
+      case 1: return f(VC<1>{});
+      case 2: return f(VC<2>{});
+      default: return f(VC<0>{});
+    }
+  }
+}
+
 template <typename R, int N>
 struct Launch {
   using G = Geo<N>;
-  static int inv_x(int mode, const cpx<R>* su, const cpx<R>* sb, cpx<R>* out, const cpx<R>* tw, R ks,
+  static int inv_x(int mode, int var, const cpx<R>* su, const cpx<R>* sb, cpx<R>* out, const cpx<R>* tw, R ks,
                    hipStream_t s) {
-    dim3 grid(cdiv(G::KZ, CfgXY<N>::L), G::K), blk(CfgXY<N>::NT);
-    switch (mode) {
-      case XM_HYDRO: hipLaunchKernelGGL((k_inv_x<R, N, XM_HYDRO>), grid, blk, 0, s, su, sb, out, tw, ks); break;
-      case XM_MHD: hipLaunchKernelGGL((k_inv_x<R, N, XM_MHD>), grid, blk, 0, s, su, sb, out, tw, ks); break;
-      case XM_COPY3: hipLaunchKernelGGL((k_inv_x<R, N, XM_COPY3>), grid, blk, 0, s, su, sb, out, tw, ks); break;
-      case XM_CURL3: hipLaunchKernelGGL((k_inv_x<R, N, XM_CURL3>), grid, blk, 0, s, su, sb, out, tw, ks); break;
-      default: hipLaunchKernelGGL((k_inv_x<R, N, XM_DIV>), grid, blk, 0, s, su, sb, out, tw, ks); break;
-    }
-    TG_LAUNCHED();
-    return TG_OK;
+    return with_variant<N>(var, [&](auto vc) {
+      constexpr int V = decltype(vc)::value;
+      using C = CfgXY<R, N, V>;
+      dim3 grid(cdiv(G::KZ, C::L), G::K), blk(C::NT);
+      switch (mode) {
+        case XM_HYDRO: hipLaunchKernelGGL((k_inv_x<R, N, XM_HYDRO, V>), grid, blk, 0, s, su, sb, out, tw, ks); break;
+        case XM_MHD: hipLaunchKernelGGL((k_inv_x<R, N, XM_MHD, V>), grid, blk, 0, s, su, sb, out, tw, ks); break;
+        case XM_COPY3: hipLaunchKernelGGL((k_inv_x<R, N, XM_COPY3, V>), grid, blk, 0, s, su, sb, out, tw, ks); break;
+        case XM_CURL3: hipLaunchKernelGGL((k_inv_x<R, N, XM_CURL3, V>), grid, blk, 0, s, su, sb, out, tw, ks); break;
+        default: hipLaunchKernelGGL((k_inv_x<R, N, XM_DIV, V>), grid, blk, 0, s, su, sb, out, tw, ks); break;
+      }
+      TG_LAUNCHED();
+      return TG_OK;
+    });
   }
-  static int y(int dir, int nf, const cpx<R>* in, cpx<R>* out, const cpx<R>* tw, hipStream_t s) {
-    dim3 grid(cdiv(G::KZ, CfgXY<N>::L), N, nf), blk(CfgXY<N>::NT);
-    if (dir > 0) hipLaunchKernelGGL((k_y<R, N, +1>), grid, blk, 0, s, in, out, tw);
-    else hipLaunchKernelGGL((k_y<R, N, -1>), grid, blk, 0, s, in, out, tw);
-    TG_LAUNCHED();
-    return TG_OK;
+  static int y(int dir, int var, int nf, const cpx<R>* in, cpx<R>* out, const cpx<R>* tw, hipStream_t s) {
+    return with_variant<N>(var, [&](auto vc) {
+      constexpr int V = decltype(vc)::value;
+      using C = CfgXY<R, N, V>;
+      dim3 grid(cdiv(G::KZ, C::L), N, nf), blk(C::NT);
+      if (dir > 0) hipLaunchKernelGGL((k_y<R, N, +1, V>), grid, blk, 0, s, in, out, tw);
+      else hipLaunchKernelGGL((k_y<R, N, -1, V>), grid, blk, 0, s, in, out, tw);
+      TG_LAUNCHED();
+      return TG_OK;
+    });
   }
-  template <int MODE>
+  template <int MODE, int V>
   static int z_(const cpx<R>* in, cpx<R>* out, const R* pin, R* pout, const cpx<R>* tw, R lor, hipStream_t s) {
-    using Z = CfgZM<N, MODE>;
+    using Z = CfgZM<R, N, MODE, V>;
     dim3 grid(N / Z::LZ, N), blk(Z::NT);
-    hipLaunchKernelGGL((k_z<R, N, MODE>), grid, blk, 0, s, in, out, pin, pout, tw, lor);
+    hipLaunchKernelGGL((k_z<R, N, MODE, V>), grid, blk, 0, s, in, out, pin, pout, tw, lor);
     TG_LAUNCHED();
     return TG_OK;
   }
-  static int z(int mode, const cpx<R>* in, cpx<R>* out, const R* pin, R* pout, const cpx<R>* tw, R lor,
+  // RHS modes take the tuned variant; edge modes (physical I/O, max |div|)
+  // always run variant 0.
+  static int z(int mode, int var, const cpx<R>* in, cpx<R>* out, const R* pin, R* pout, const cpx<R>* tw, R lor,
                hipStream_t s) {
     switch (mode) {
-      case ZM_HYDRO: return z_<ZM_HYDRO>(in, out, pin, pout, tw, lor, s);
-      case ZM_MHD: return z_<ZM_MHD>(in, out, pin, pout, tw, lor, s);
-      case ZM_PHYS_OUT3: return z_<ZM_PHYS_OUT3>(in, out, pin, pout, tw, lor, s);
-      case ZM_MAXABS: return z_<ZM_MAXABS>(in, out, pin, pout, tw, lor, s);
-      case ZM_PHYS_OUT1: return z_<ZM_PHYS_OUT1>(in, out, pin, pout, tw, lor, s);
-      default: return z_<ZM_PHYS_IN3>(in, out, pin, pout, tw, lor, s);
+      case ZM_HYDRO:
+        return with_variant<N>(var, [&](auto vc) {
+          return z_<ZM_HYDRO, decltype(vc)::value>(in, out, pin, pout, tw, lor, s);
+        });
+      case ZM_MHD:
+        return with_variant<N>(var, [&](auto vc) {
+          return z_<ZM_MHD, decltype(vc)::value>(in, out, pin, pout, tw, lor, s);
+        });
+      case ZM_PHYS_OUT3: return z_<ZM_PHYS_OUT3, 0>(in, out, pin, pout, tw, lor, s);
+      case ZM_MAXABS: return z_<ZM_MAXABS, 0>(in, out, pin, pout, tw, lor, s);
+      case ZM_PHYS_OUT1: return z_<ZM_PHYS_OUT1, 0>(in, out, pin, pout, tw, lor, s);
+      default: return z_<ZM_PHYS_IN3, 0>(in, out, pin, pout, tw, lor, s);
     }
   }
-  static int nmax() { return (N / CfgZM<N, ZM_MAXABS>::LZ) * N; }
-  static int fwd_x(int mode, const cpx<R>* in, const StageArgs<R>& a, const cpx<R>* tw, hipStream_t s) {
-    const int ff = (mode == FM_MHD_STAGE || mode == FM_MHD_RHS) ? 6 : 3;
-    const int L = CfgXF<N>::LINES / ff;
-    dim3 grid(cdiv(G::KZ, L), G::K), blk(CfgXF<N>::NT);
-    switch (mode) {
-      case FM_HYDRO_STAGE: hipLaunchKernelGGL((k_fwd_x<R, N, FM_HYDRO_STAGE>), grid, blk, 0, s, in, a, tw); break;
-      case FM_MHD_STAGE: hipLaunchKernelGGL((k_fwd_x<R, N, FM_MHD_STAGE>), grid, blk, 0, s, in, a, tw); break;
-      case FM_STORE3: hipLaunchKernelGGL((k_fwd_x<R, N, FM_STORE3>), grid, blk, 0, s, in, a, tw); break;
-      case FM_HYDRO_RHS: hipLaunchKernelGGL((k_fwd_x<R, N, FM_HYDRO_RHS>), grid, blk, 0, s, in, a, tw); break;
-      default: hipLaunchKernelGGL((k_fwd_x<R, N, FM_MHD_RHS>), grid, blk, 0, s, in, a, tw); break;
-    }
-    TG_LAUNCHED();
-    return TG_OK;
+  static int nmax() { return (N / CfgZM<R, N, ZM_MAXABS, 0>::LZ) * N; }
+  static int fwd_x(int mode, int var, const cpx<R>* in, const StageArgs<R>& a, const cpx<R>* tw, hipStream_t s) {
+    return with_variant<N>(var, [&](auto vc) {
+      constexpr int V = decltype(vc)::value;
+      using C = CfgXF<R, N, V>;
+      const int ff = (mode == FM_MHD_STAGE || mode == FM_MHD_RHS) ? 6 : 3;
+      const int L = C::LINES / ff;
+      dim3 grid(cdiv(G::KZ, L), G::K), blk(C::NT);
+      switch (mode) {
+        case FM_HYDRO_STAGE: hipLaunchKernelGGL((k_fwd_x<R, N, FM_HYDRO_STAGE, V>), grid, blk, 0, s, in, a, tw); break;
+        case FM_MHD_STAGE: hipLaunchKernelGGL((k_fwd_x<R, N, FM_MHD_STAGE, V>), grid, blk, 0, s, in, a, tw); break;
+        case FM_STORE3: hipLaunchKernelGGL((k_fwd_x<R, N, FM_STORE3, V>), grid, blk, 0, s, in, a, tw); break;
+        case FM_HYDRO_RHS: hipLaunchKernelGGL((k_fwd_x<R, N, FM_HYDRO_RHS, V>), grid, blk, 0, s, in, a, tw); break;
+        default: hipLaunchKernelGGL((k_fwd_x<R, N, FM_MHD_RHS, V>), grid, blk, 0, s, in, a, tw); break;
+      }
+      TG_LAUNCHED();
+      return TG_OK;
+    });
   }
   static int diag(const cpx<R>* su, const cpx<R>* sb, double* part, int nblk, double ks, hipStream_t s) {
     hipLaunchKernelGGL((k_diag<R, N>), dim3(nblk), dim3(DIAG_NT), 0, s, su, sb, part, ks);
@@ -202,11 +241,12 @@ int rhs_passes(const tg_plan* p, const Ptrs<R>& w, const cpx<R>* us, const cpx<R
   const cpx<R>* tw = static_cast<const cpx<R>*>(p->tw);
   const bool mhd = p->mhd != 0;
   const int fi = mhd ? 12 : 6, fo = mhd ? 6 : 3;
-  TG_TIMED(TG_PASS_INV_X, Lc::inv_x(mhd ? XM_MHD : XM_HYDRO, us, bs, w.A, tw, R(p->ks), s));
-  TG_TIMED(TG_PASS_INV_Y, Lc::y(+1, fi, w.A, w.B, tw, s));
-  TG_TIMED(TG_PASS_Z, Lc::z(mhd ? ZM_MHD : ZM_HYDRO, w.B, w.A, nullptr, nullptr, tw, lor, s));
-  TG_TIMED(TG_PASS_FWD_Y, Lc::y(-1, fo, w.A, w.B, tw, s));
-  TG_TIMED(TG_PASS_FWD_X, Lc::fwd_x(fmode, w.B, a, tw, s));
+  const int* v = p->var;
+  TG_TIMED(TG_PASS_INV_X, Lc::inv_x(mhd ? XM_MHD : XM_HYDRO, v[TG_PASS_INV_X], us, bs, w.A, tw, R(p->ks), s));
+  TG_TIMED(TG_PASS_INV_Y, Lc::y(+1, v[TG_PASS_INV_Y], fi, w.A, w.B, tw, s));
+  TG_TIMED(TG_PASS_Z, Lc::z(mhd ? ZM_MHD : ZM_HYDRO, v[TG_PASS_Z], w.B, w.A, nullptr, nullptr, tw, lor, s));
+  TG_TIMED(TG_PASS_FWD_Y, Lc::y(-1, v[TG_PASS_FWD_Y], fo, w.A, w.B, tw, s));
+  TG_TIMED(TG_PASS_FWD_X, Lc::fwd_x(fmode, v[TG_PASS_FWD_X], w.B, a, tw, s));
   return TG_OK;
 }
 
@@ -216,12 +256,12 @@ struct Ops {
     using Lc = Launch<R, N>;
     Ptrs<R> w = carve<R>(p, ws);
     const cpx<R>* tw = static_cast<const cpx<R>*>(p->tw);
-    TG_RET(Lc::z(ZM_PHYS_IN3, nullptr, w.A, phys, nullptr, tw, R(0), s));
-    TG_RET(Lc::y(-1, 3, w.A, w.B, tw, s));
+    TG_RET(Lc::z(ZM_PHYS_IN3, 0, nullptr, w.A, phys, nullptr, tw, R(0), s));
+    TG_RET(Lc::y(-1, 0, 3, w.A, w.B, tw, s));
     StageArgs<R> a{};
     a.unext = spec;
     a.ks = R(p->ks);
-    return Lc::fwd_x(FM_STORE3, w.B, a, tw, s);
+    return Lc::fwd_x(FM_STORE3, 0, w.B, a, tw, s);
   }
   static int inverse(const tg_plan* p, const cpx<R>* spec, int op, R* phys, void* ws, hipStream_t s) {
     using Lc = Launch<R, N>;
@@ -229,9 +269,9 @@ struct Ops {
     const cpx<R>* tw = static_cast<const cpx<R>*>(p->tw);
     const int xm = op == TG_OP_CURL ? XM_CURL3 : op == TG_OP_DIV ? XM_DIV : XM_COPY3;
     const int nf = op == TG_OP_DIV ? 1 : 3;
-    TG_RET(Lc::inv_x(xm, spec, nullptr, w.A, tw, R(p->ks), s));
-    TG_RET(Lc::y(+1, nf, w.A, w.B, tw, s));
-    return Lc::z(op == TG_OP_DIV ? ZM_PHYS_OUT1 : ZM_PHYS_OUT3, w.B, nullptr, nullptr, phys, tw, R(0), s);
+    TG_RET(Lc::inv_x(xm, 0, spec, nullptr, w.A, tw, R(p->ks), s));
+    TG_RET(Lc::y(+1, 0, nf, w.A, w.B, tw, s));
+    return Lc::z(op == TG_OP_DIV ? ZM_PHYS_OUT1 : ZM_PHYS_OUT3, 0, w.B, nullptr, nullptr, phys, tw, R(0), s);
   }
   static int rhs(const tg_plan* p, const cpx<R>* u, const cpx<R>* b, double nu, double eta, double lor,
                  cpx<R>* ru, cpx<R>* rb, void* ws, hipStream_t s) {
@@ -275,13 +315,13 @@ struct Ops {
     int nm = 0;
     if (divmax) {
       nm = p->nMax;
-      TG_RET(Lc::inv_x(XM_DIV, u, nullptr, w.A, tw, R(p->ks), s));
-      TG_RET(Lc::y(+1, 1, w.A, w.B, tw, s));
-      TG_RET(Lc::z(ZM_MAXABS, w.B, nullptr, nullptr, w.maxu, tw, R(0), s));
+      TG_RET(Lc::inv_x(XM_DIV, 0, u, nullptr, w.A, tw, R(p->ks), s));
+      TG_RET(Lc::y(+1, 0, 1, w.A, w.B, tw, s));
+      TG_RET(Lc::z(ZM_MAXABS, 0, w.B, nullptr, nullptr, w.maxu, tw, R(0), s));
       if (b) {
-        TG_RET(Lc::inv_x(XM_DIV, b, nullptr, w.A, tw, R(p->ks), s));
-        TG_RET(Lc::y(+1, 1, w.A, w.B, tw, s));
-        TG_RET(Lc::z(ZM_MAXABS, w.B, nullptr, nullptr, w.maxb, tw, R(0), s));
+        TG_RET(Lc::inv_x(XM_DIV, 0, b, nullptr, w.A, tw, R(p->ks), s));
+        TG_RET(Lc::y(+1, 0, 1, w.A, w.B, tw, s));
+        TG_RET(Lc::z(ZM_MAXABS, 0, w.B, nullptr, nullptr, w.maxb, tw, R(0), s));
       }
     }
     hipLaunchKernelGGL((k_diag_final<R>), dim3(1), dim3(64), 0, s, w.part, p->nDiagBlk,
@@ -510,6 +550,20 @@ int tg_diagnostics(tg_plan* p, const void* u_hat, const void* b_hat, int with_di
     return decltype(ops)::diagnostics(p, static_cast<const cpx<R>*>(u_hat), static_cast<const cpx<R>*>(b_hat),
                                       with_divmax, out, ws, static_cast<hipStream_t>(stream));
   });
+}
+
+int tg_plan_variants(tg_plan* p, const int* set, int* get, int* count) {
+  TG_RET(check_plan(p));
+  const int nv = p->n >= 128 ? NVAR : 1;
+  if (count) *count = nv;
+  if (set) {
+    for (int i = 0; i < TG_NPASS; ++i)
+      if (set[i] < 0 || set[i] >= nv) return fail(TG_ERR_ARG, "tg_plan_variants: variant out of range");
+    for (int i = 0; i < TG_NPASS; ++i) p->var[i] = set[i];
+  }
+  if (get)
+    for (int i = 0; i < TG_NPASS; ++i) get[i] = p->var[i];
+  return TG_OK;
 }
 
 int tg_timing_enable(tg_plan* p, int on) {

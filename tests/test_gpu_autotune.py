@@ -1,0 +1,58 @@
+"""Every compiled tile variant computes the same RK4 step (to round-off of
+the different radix plans), and the autotuner picks valid variants and
+caches them."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import tg_oracle as o
+from tests.helpers import band_limited_field, rel_l2
+
+pytestmark = pytest.mark.gpu
+L = 2 * np.pi
+
+
+@pytest.mark.parametrize("mhd", [False, True])
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_all_variants_agree(dtype, mhd):
+    from jax_tg_project_amd._lib import Plan
+    N = 128
+    p = Plan(N, dtype=dtype, mhd=mhd)
+    assert p.variant_count() == 3
+    u = o.initial_taylor_green(N) + 0.1 * band_limited_field(N, 2)
+    ut = torch.tensor(u, dtype=dtype, device="cuda")
+    outs = []
+    for v in range(3):
+        for passv in ([v] * 5, [v, (v + 1) % 3, (v + 2) % 3, v, (v + 1) % 3]):
+            p.set_variants(passv)
+            uh = p.forward(ut)
+            bh = p.forward((0.3 * ut).contiguous()) if mhd else None
+            p.rk4(uh, bh, nsteps=2, dt=2e-3, nu=1 / 400, eta=1 / 400, lorentz_sign=1.0)
+            outs.append(p.inverse(uh).cpu().numpy())
+    tol = 1e-13 if dtype == torch.float64 else 2e-6
+    for x in outs[1:]:
+        assert rel_l2(x, outs[0]) < tol
+    if dtype == torch.float64 and not mhd:
+        kx, ky, kz = o.make_wavenumbers(N, L)
+        m = o.make_dealias_mask(N, L)
+        ref = u
+        for _ in range(2):
+            ref = o.advance_one_step(ref, 2e-3, 1 / 400, kx, ky, kz, m)
+        assert rel_l2(outs[-1], ref) < 1e-12
+
+
+def test_tuner_selects_and_caches(tmp_path, monkeypatch):
+    from jax_tg_project_amd import autotune
+    from jax_tg_project_amd._lib import Plan
+    monkeypatch.setenv(autotune.CACHE_ENV, str(tmp_path / "tune.json"))
+    p = Plan(128, dtype=torch.float32)
+    best, ms = autotune.tune(p, steps=2)
+    assert len(best) == 5 and all(0 <= v < 3 for v in best)
+    assert p.get_variants() == best
+    for name, times in ms.items():
+        assert times[best[list(ms).index(name)]] == min(times)
+    q = Plan(128, dtype=torch.float32)
+    again, _ = autotune.tune(q)
+    assert again == best  # served from the cache
+    small = Plan(32)
+    assert autotune.tune(small)[0] == [0] * 5
