@@ -236,7 +236,16 @@ __device__ __forceinline__ void fft_apply(cpx<R>* buf, int LP, int l, int t, boo
 // no barrier is needed before this stage writes LDS), 2 = functor that reads
 // LDS itself (barrier before writing).  DST_GLOBAL: results go to the store
 // functor instead of LDS (then no trailing barrier).
-template <typename R, int N, int E, int DIR, int P, int RAD, int SRC, bool DST_GLOBAL, class Ld, class St>
+// LDS index of element n of line l with one pad element every 2^PS rows
+// (PS = 0: no padding).  The pad keeps a radix-R first stage that writes at
+// stride R*LP from landing every lane of a wave on the same bank.
+template <int PS>
+__device__ __forceinline__ int lidx(int n, int l, int LP) {
+  if constexpr (PS > 0) return n * LP + (n >> PS) + l;
+  else return n * LP + l;
+}
+
+template <typename R, int N, int E, int DIR, int P, int RAD, int SRC, bool DST_GLOBAL, int PS, class Ld, class St>
 __device__ __forceinline__ void io_stage(cpx<R>* buf, int LP, int l, int t, bool active,
                                          const cpx<R>* __restrict__ tw, const Ld& ld, const St& st) {
   static_assert(E % RAD == 0, "radix must divide points per thread");
@@ -250,7 +259,7 @@ __device__ __forceinline__ void io_stage(cpx<R>* buf, int LP, int l, int t, bool
 #pragma unroll
       for (int m = 0; m < RAD; ++m) {
         const int n = b + m * SIN;
-        if constexpr (SRC == 0) v[j][m] = buf[n * LP + l];
+        if constexpr (SRC == 0) v[j][m] = buf[lidx<PS>(n, l, LP)];
         else v[j][m] = ld(n, l);
       }
     }
@@ -274,29 +283,40 @@ __device__ __forceinline__ void io_stage(cpx<R>* buf, int LP, int l, int t, bool
 #pragma unroll
       for (int m = 0; m < RAD; ++m) {
         if constexpr (DST_GLOBAL) st(base + m * P, l, v[j][m]);
-        else buf[(base + m * P) * LP + l] = v[j][m];
+        else buf[lidx<PS>(base + m * P, l, LP)] = v[j][m];
       }
     }
   }
   if constexpr (!DST_GLOBAL) __syncthreads();
 }
 
-template <typename R, int N, int E, int DIR, int SRC, bool DST_GLOBAL, int P, class Ld, class St, int R0,
-          int... Rest>
+template <typename R, int N, int E, int DIR, int SRC, bool DST_GLOBAL, int PS, int P, class Ld, class St,
+          int R0, int... Rest>
 __device__ __forceinline__ void io_stages(cpx<R>* buf, int LP, int l, int t, bool active,
                                           const cpx<R>* __restrict__ tw, const Ld& ld, const St& st) {
   constexpr bool last = sizeof...(Rest) == 0;
-  io_stage<R, N, E, DIR, P, R0, SRC, last && DST_GLOBAL>(buf, LP, l, t, active, tw, ld, st);
+  io_stage<R, N, E, DIR, P, R0, SRC, last && DST_GLOBAL, PS>(buf, LP, l, t, active, tw, ld, st);
   if constexpr (!last)
-    io_stages<R, N, E, DIR, 0, DST_GLOBAL, P * R0, Ld, St, Rest...>(buf, LP, l, t, active, tw, ld, st);
+    io_stages<R, N, E, DIR, 0, DST_GLOBAL, PS, P * R0, Ld, St, Rest...>(buf, LP, l, t, active, tw, ld, st);
 }
 
-template <typename R, int N, int E, int DIR, int SRC, bool DST_GLOBAL, class Ld, class St, int... Rs>
+template <typename R, int N, int E, int DIR, int SRC, bool DST_GLOBAL, int PS, class Ld, class St, int... Rs>
 __device__ __forceinline__ void io_apply(cpx<R>* buf, int LP, int l, int t, bool active,
                                          const cpx<R>* __restrict__ tw, const Ld& ld, const St& st,
                                          Radices<Rs...>) {
-  io_stages<R, N, E, DIR, SRC, DST_GLOBAL, 1, Ld, St, Rs...>(buf, LP, l, t, active, tw, ld, st);
+  io_stages<R, N, E, DIR, SRC, DST_GLOBAL, PS, 1, Ld, St, Rs...>(buf, LP, l, t, active, tw, ld, st);
 }
+
+template <class Plan>
+struct FirstRadix;
+template <int R0, int... Rest>
+struct FirstRadix<Radices<R0, Rest...>> {
+  static constexpr int value = R0;
+};
+__host__ __device__ constexpr int ilog2(int v) { return v <= 1 ? 0 : 1 + ilog2(v / 2); }
+// Pad shift for a slice-fastest transform of length N with E points per thread.
+template <int N, int E>
+constexpr int pad_shift() { return ilog2(FirstRadix<typename PlanFor<N, E>::type>::value); }
 
 // Transform `nlines` lines whose first stage reads through `ld(n, l)` and
 // (if DST_GLOBAL) whose last stage writes through `st(n, l, v)` in natural
@@ -308,14 +328,28 @@ struct NoIO {
   __device__ void operator()(A...) const {}
 };
 
-template <typename R, int N, int E, int DIR, int SRC, bool DST_GLOBAL, class Ld, class St>
+// Copy the N-entry twiddle table to LDS.  No barrier here: a first stage
+// never reads twiddles (P = 1), and the barrier that ends it publishes the
+// table to every later stage.
+template <typename R, int N>
+__device__ __forceinline__ void stage_twiddles(cpx<R>* tws, const cpx<R>* __restrict__ tw) {
+  for (int i = threadIdx.x; i < N; i += blockDim.x) tws[i] = tw[i];
+}
+
+// TFAST selects the thread mapping: false = consecutive threads take
+// consecutive lines (coalesced when lines are adjacent columns in memory),
+// true = consecutive threads take consecutive slices of one line (coalesced
+// when a line is contiguous in memory).
+template <typename R, int N, int E, int DIR, int SRC, bool DST_GLOBAL, bool TFAST = false, int PS = 0, class Ld,
+          class St>
 __device__ __forceinline__ void fft_io(cpx<R>* buf, int LP, int nlines, const cpx<R>* __restrict__ tw,
                                        const Ld& ld, const St& st) {
   const int tid = threadIdx.x;
-  const int l = tid % nlines;
-  const int t = tid / nlines;
-  const bool active = t < N / E;
-  io_apply<R, N, E, DIR, SRC, DST_GLOBAL>(buf, LP, l, t, active, tw, ld, st, typename PlanFor<N, E>::type{});
+  constexpr int TPL = N / E;
+  const int l = TFAST ? tid / TPL : tid % nlines;
+  const int t = TFAST ? tid % TPL : tid / nlines;
+  const bool active = TFAST ? (l < nlines) : (t < TPL);
+  io_apply<R, N, E, DIR, SRC, DST_GLOBAL, PS>(buf, LP, l, t, active, tw, ld, st, typename PlanFor<N, E>::type{});
 }
 
 // Transform `nlines` lines of length N held in LDS (buf[n*LP + l]).  The

@@ -114,23 +114,25 @@ __global__ __launch_bounds__((CfgXY<R, N, V>::NT)) void k_inv_x(const cpx<R>* __
   const size_t cs = size_t(G::K) * G::K * G::KZ;
   const R scale = R(1) / (R(N) * R(N) * R(N));
   const R ky = ks * R(G::kint(iy));
+  __shared__ cpx<R> tws[N];
+  stage_twiddles<R, N>(tws, tw);
   for (int f = 0; f < F; ++f) {
     if (f) __syncthreads();  // the previous field's last LDS reads are done
     auto ld = [&](int n, int ll) -> cpx<R> {
-      const int iz = iz0 + ll;
-      cpx<R> v = {R(0), R(0)};
-      if (iz < G::KZ && G::kept(n)) {
-        const int ix = G::compact_of(n);
-        const size_t pt = (size_t(ix) * G::K + iy) * G::KZ + iz;
-        v = cscale(xfield<R, N, MODE>(f, su, sb, pt, cs, ks * R(G::kint(ix)), ky, ks * R(iz)), scale);
-      }
-      return v;
+      // branch-free: clamp to a valid address, load, then select zero
+      const int iz0c = iz0 + ll;
+      const bool ok = iz0c < G::KZ && G::kept(n);
+      const int iz = ok ? iz0c : 0;
+      const int ix = ok ? G::compact_of(n) : 0;
+      const size_t pt = (size_t(ix) * G::K + iy) * G::KZ + iz;
+      const cpx<R> v = cscale(xfield<R, N, MODE>(f, su, sb, pt, cs, ks * R(G::kint(ix)), ky, ks * R(iz)), scale);
+      return ok ? v : cpx<R>{R(0), R(0)};
     };
     auto st = [&](int n, int ll, cpx<R> v) {
       const int iz = iz0 + ll;
       if (iz < G::KZ) out[((size_t(f) * N + n) * G::K + iy) * G::KZ + iz] = v;
     };
-    fft_io<R, N, E, +1, 1, true>(buf, L, L, tw, ld, st);
+    fft_io<R, N, E, +1, 1, true>(buf, L, L, tws, ld, st);
   }
 }
 
@@ -148,17 +150,20 @@ __global__ __launch_bounds__((CfgXY<R, N, V>::NT)) void k_y(const cpx<R>* __rest
   const int iz0 = blockIdx.x * L, x = blockIdx.y, f = blockIdx.z;
   const int tid = threadIdx.x;
   const size_t plane = size_t(f) * N + x;
+  __shared__ cpx<R> tws[N];
+  stage_twiddles<R, N>(tws, tw);
   auto ld = [&](int n, int ll) -> cpx<R> {
-    const int iz = iz0 + ll;
-    cpx<R> v = {R(0), R(0)};
-    if (iz < G::KZ) {
-      if constexpr (DIR > 0) {
-        if (G::kept(n)) v = in[(plane * G::K + G::compact_of(n)) * G::KZ + iz];
-      } else {
-        v = in[(plane * N + n) * G::KZ + iz];
-      }
+    const int izr = iz0 + ll;
+    bool ok = izr < G::KZ;
+    const int iz = ok ? izr : 0;
+    cpx<R> v;
+    if constexpr (DIR > 0) {
+      ok = ok && G::kept(n);
+      v = in[(plane * G::K + (ok ? G::compact_of(n) : 0)) * G::KZ + iz];
+    } else {
+      v = in[(plane * N + n) * G::KZ + iz];
     }
-    return v;
+    return ok ? v : cpx<R>{R(0), R(0)};
   };
   auto st = [&](int n, int ll, cpx<R> v) {
     const int iz = iz0 + ll;
@@ -169,7 +174,7 @@ __global__ __launch_bounds__((CfgXY<R, N, V>::NT)) void k_y(const cpx<R>* __rest
       if (G::kept(n)) out[(plane * G::K + G::compact_of(n)) * G::KZ + iz] = v;
     }
   };
-  fft_io<R, N, E, DIR, 1, true>(buf, L, L, tw, ld, st);
+  fft_io<R, N, E, DIR, 1, true>(buf, L, L, tws, ld, st);
 }
 
 // ----------------------------------------------------------------- pass 3
@@ -216,7 +221,15 @@ __global__ __launch_bounds__((CfgZM<R, N, MODE, V>::NT)) void k_z(const cpx<R>* 
   constexpr int LZ = Z::LZ;  // lines per block
   constexpr int KZ = G::KZ;
   static_assert(N % LZ == 0, "lines per block must divide N");
-  __shared__ cpx<R> buf[N * LP];
+  // padded LDS layouts of the inverse output and of the forward input
+  constexpr int PSI = pad_shift<N, E>();
+  constexpr int PSF = (S::FOUT > 0) ? pad_shift<N, E / 2>() : PSI;
+  constexpr int PSMIN = PSI < PSF ? PSI : PSF;
+  __shared__ cpx<R> buf[N * LP + (N >> PSMIN) + 1];
+  auto bi = [&](int n, int l) -> cpx<R>& { return buf[lidx<PSI>(n, l, LP)]; };
+  __shared__ cpx<R> tws[N];
+  stage_twiddles<R, N>(tws, tw);
+  auto bf = [&](int n, int l) -> cpx<R>& { return buf[lidx<PSF>(n, l, LP)]; };
   const int tid = threadIdx.x;
   const int x = blockIdx.y, y0 = blockIdx.x * LZ;
   const size_t NNKZ = size_t(N) * N * KZ;
@@ -233,31 +246,32 @@ __global__ __launch_bounds__((CfgZM<R, N, MODE, V>::NT)) void k_z(const cpx<R>* 
       else if constexpr (MODE == ZM_PHYS_OUT3) { fa = 2 * p; fb = (2 * p + 1 < 3) ? 2 * p + 1 : -1; }
       else { fa = 0; fb = -1; }
       const bool neg = k > N / 2;
-      const int kz = neg ? N - k : k;
-      cpx<R> a = {R(0), R(0)}, b = {R(0), R(0)};
-      if (kz < KZ) {
-        const size_t off = (line0 + line) * KZ + kz;
-        a = in[fa * NNKZ + off];
-        if (fb >= 0) b = in[fb * NNKZ + off];
-        if (k == 0) { a.y = R(0); b.y = R(0); }
-        if (neg) { a.y = -a.y; b.y = -b.y; }
-      }
-      return {a.x - b.y, a.y + b.x};
+      const int kzr = neg ? N - k : k;
+      const bool ok = kzr < KZ;
+      const int kz = ok ? kzr : 0;  // branch-free: clamped load, then select
+      const size_t off = (line0 + line) * KZ + kz;
+      cpx<R> a = in[fa * NNKZ + off];
+      cpx<R> b = in[(fb >= 0 ? fb : fa) * NNKZ + off];
+      if (fb < 0) b = {R(0), R(0)};
+      if (k == 0) { a.y = R(0); b.y = R(0); }
+      if (neg) { a.y = -a.y; b.y = -b.y; }
+      const cpx<R> z = {a.x - b.y, a.y + b.x};
+      return ok ? z : cpx<R>{R(0), R(0)};
     };
-    fft_io<R, N, E, +1, 1, false>(buf, LP, NSEQ, tw, ld, NoIO{});
+    fft_io<R, N, E, +1, 1, false, true, PSI>(buf, LP, NSEQ, tws, ld, NoIO{});
   }
 
   if constexpr (MODE == ZM_PHYS_OUT3) {
     for (int idx = tid; idx < LZ * 3 * N; idx += NT) {
       const int z = idx % N, f = (idx / N) % 3, line = idx / (3 * N);
-      const cpx<R> v = buf[z * LP + line * S::SPL + f / 2];
+      const cpx<R> v = bi(z, line * S::SPL + f / 2);
       phys_out[(size_t(f) * N * N + line0 + line) * N + z] = (f & 1) ? v.y : v.x;
     }
     return;
   } else if constexpr (MODE == ZM_PHYS_OUT1) {
     for (int idx = tid; idx < LZ * N; idx += NT) {
       const int z = idx % N, line = idx / N;
-      phys_out[(line0 + line) * N + z] = buf[z * LP + line].x;
+      phys_out[(line0 + line) * N + z] = bi(z, line).x;
     }
     return;
   } else if constexpr (MODE == ZM_MAXABS) {
@@ -265,7 +279,7 @@ __global__ __launch_bounds__((CfgZM<R, N, MODE, V>::NT)) void k_z(const cpx<R>* 
     R m = R(0);
     for (int idx = tid; idx < LZ * N; idx += NT) {
       const int z = idx % N, line = idx / N;
-      m = fmax(m, fabs(buf[z * LP + line].x));
+      m = fmax(m, fabs(bi(z, line).x));
     }
     red[tid] = m;
     __syncthreads();
@@ -296,7 +310,7 @@ __global__ __launch_bounds__((CfgZM<R, N, MODE, V>::NT)) void k_z(const cpx<R>* 
           R u[3], w[3];
 #pragma unroll
           for (int p = 0; p < 3; ++p) {
-            const cpx<R> v = buf[z * LP + line * 3 + p];
+            const cpx<R> v = bi(z, line * 3 + p);
             u[p] = v.x; w[p] = v.y;
           }
           cross3(u, w, &res[it][3 * h]);
@@ -305,8 +319,8 @@ __global__ __launch_bounds__((CfgZM<R, N, MODE, V>::NT)) void k_z(const cpx<R>* 
         R u[3], w[3], b[3], j[3];
 #pragma unroll
         for (int p = 0; p < 3; ++p) {
-          const cpx<R> v = buf[z * LP + g * 6 + p];
-          const cpx<R> q = buf[z * LP + g * 6 + 3 + p];
+          const cpx<R> v = bi(z, g * 6 + p);
+          const cpx<R> q = bi(z, g * 6 + 3 + p);
           u[p] = v.x; w[p] = v.y; b[p] = q.x; j[p] = q.y;
         }
         R uw[3], jb[3];
@@ -334,23 +348,23 @@ __global__ __launch_bounds__((CfgZM<R, N, MODE, V>::NT)) void k_z(const cpx<R>* 
       const int z = item % N, g = item / N;
       const R* r = res[it];
       if constexpr (MODE == ZM_MHD) {
-        buf[z * LP + 3 * g + 0] = {r[0], r[1]};
-        buf[z * LP + 3 * g + 1] = {r[2], r[3]};
-        buf[z * LP + 3 * g + 2] = {r[4], r[5]};
+        bf(z, 3 * g + 0) = {r[0], r[1]};
+        bf(z, 3 * g + 1) = {r[2], r[3]};
+        bf(z, 3 * g + 2) = {r[4], r[5]};
       } else {
-        buf[z * LP + 3 * g + 0] = {r[0], r[1]};
-        buf[z * LP + 3 * g + 1] = {r[3], r[4]};
-        buf[z * LP + 3 * g + 2] = {r[2], r[5]};
+        bf(z, 3 * g + 0) = {r[0], r[1]};
+        bf(z, 3 * g + 1) = {r[3], r[4]};
+        bf(z, 3 * g + 2) = {r[2], r[5]};
       }
     }
     __syncthreads();
-    fft_lines<R, N, E / 2, -1>(buf, LP, NSEQ / 2, tw);
+    fft_io<R, N, E / 2, -1, 0, false, true, PSF>(buf, LP, NSEQ / 2, tws, NoIO{}, NoIO{});
     // ---- unpack the kept kz of each pair and store
     constexpr int NQ = NSEQ / 2;
     for (int idx = tid; idx < NQ * KZ; idx += NT) {
       const int kz = idx % KZ, q = idx / KZ;
-      const cpx<R> P = buf[kz * LP + q];
-      const cpx<R> Qc = buf[((N - kz) % N) * LP + q];  // conj taken below
+      const cpx<R> P = bf(kz, q);
+      const cpx<R> Qc = bf((N - kz) % N, q);  // conj taken below
       const cpx<R> re = {R(0.5) * (P.x + Qc.x), R(0.5) * (P.y - Qc.y)};
       const cpx<R> im = {R(0.5) * (P.y + Qc.y), R(0.5) * (Qc.x - P.x)};
       const int g = q / 3, p = q % 3;
@@ -432,13 +446,15 @@ __global__ __launch_bounds__((CfgXF<R, N, V>::NT)) void k_fwd_x(const cpx<R>* __
   __shared__ cpx<R> buf[N * LP];
   const int iz0 = blockIdx.x * L, iy = blockIdx.y;
   const int tid = threadIdx.x;
+  __shared__ cpx<R> tws[N];
+  stage_twiddles<R, N>(tws, tw);
   auto ld = [&](int n, int ln) -> cpx<R> {
-    const int f = ln / L, iz = iz0 + ln % L;
-    cpx<R> v = {R(0), R(0)};
-    if (iz < G::KZ) v = in[((size_t(f) * N + n) * G::K + iy) * G::KZ + iz];
-    return v;
+    const int f = ln / L, izr = iz0 + ln % L;
+    const bool ok = izr < G::KZ;
+    const cpx<R> v = in[((size_t(f) * N + n) * G::K + iy) * G::KZ + (ok ? izr : 0)];
+    return ok ? v : cpx<R>{R(0), R(0)};
   };
-  fft_io<R, N, E, -1, 1, false>(buf, LP, LINES, tw, ld, NoIO{});
+  fft_io<R, N, E, -1, 1, false>(buf, LP, LINES, tws, ld, NoIO{});
   const size_t cs = size_t(G::K) * G::K * G::KZ;
   const R ky = a.ks * R(G::kint(iy));
   for (int idx = tid; idx < G::K * L; idx += NT) {
