@@ -141,6 +141,26 @@ int tg_timing_enable(tg_plan* plan, int on);
 int tg_plan_variants(tg_plan* plan, const int* set, int* get, int* count);
 int tg_timing_read(tg_plan* plan, double* ms, long long* count);
 
+/* ---- Batched ensemble (no counterpart in the reference, which advances one
+ * instance; each instance is the advance_one_step of
+ * jax/taylor_green_jax.py:188-196 on its own grid).  Instances may have
+ * different N (16, 32, 48, 64, 96, 128) and viscosities; one launch per pass
+ * advances all of them.  Instance i's compact state (3 components, plan
+ * layout for its N) lives at complex-element offset state_offsets[i] of one
+ * caller-owned state buffer; the workspace is one buffer of ws_bytes. */
+typedef struct tg_batch tg_batch;
+int tg_batch_create(int ninst, const int* n, const double* nu, int precision_bits, double domain_length,
+                    tg_batch** out);
+int tg_batch_destroy(tg_batch* batch);
+/* state_offsets[ninst + 1] (complex elements) and the workspace size. */
+int tg_batch_query(const tg_batch* batch, long long* state_offsets, long long* ws_bytes);
+/* Record the state and workspace device pointers used by later steps. */
+int tg_batch_bind(tg_batch* batch, void* state, void* ws);
+/* nsteps classic RK4 steps of every instance with time step dt.  With
+ * use_graph != 0 one step (4 stages x 5 batched launches) is captured once
+ * into a HIP graph (again when dt changes) and replayed nsteps times. */
+int tg_batch_rk4_steps(tg_batch* batch, int nsteps, double dt, int use_graph, void* stream);
+
 /* Copy the calling thread's last error message into buf (NUL-terminated). */
 int tg_last_error(char* buf, size_t n);
 

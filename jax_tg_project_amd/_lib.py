@@ -41,6 +41,13 @@ _SIGS = {
     "tg_timing_enable": ([_VP, ctypes.c_int], ctypes.c_int),
     "tg_timing_read": ([_VP, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_longlong)],
                        ctypes.c_int),
+    "tg_batch_create": ([ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double),
+                         ctypes.c_int, ctypes.c_double, ctypes.POINTER(_VP)], ctypes.c_int),
+    "tg_batch_destroy": ([_VP], ctypes.c_int),
+    "tg_batch_query": ([_VP, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)],
+                       ctypes.c_int),
+    "tg_batch_bind": ([_VP, _VP, _VP], ctypes.c_int),
+    "tg_batch_rk4_steps": ([_VP, ctypes.c_int, ctypes.c_double, ctypes.c_int, _VP], ctypes.c_int),
     "tg_last_error": ([ctypes.c_char_p, ctypes.c_size_t], ctypes.c_int),
 }
 PASS_NAMES = ("inv_x", "inv_y", "z", "fwd_y", "fwd_x")
@@ -80,6 +87,10 @@ def supported_sizes():
     arr = (ctypes.c_int * 32)()
     n = lib.tg_supported_sizes(arr, 32)
     return tuple(arr[i] for i in range(n))
+
+
+def _check_rc(rc):
+    _check(rc)
 
 
 def _require_gpu():

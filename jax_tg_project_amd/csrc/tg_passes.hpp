@@ -100,81 +100,104 @@ __device__ __forceinline__ cpx<R> xfield(int f, const cpx<R>* __restrict__ su,
 }
 
 template <typename R, int N, int MODE, int V>
-__global__ __launch_bounds__((CfgXY<R, N, V>::NT)) void k_inv_x(const cpx<R>* __restrict__ su,
-                                                        const cpx<R>* __restrict__ sb,
-                                                        cpx<R>* __restrict__ out,
-                                                        const cpx<R>* __restrict__ tw, R ks) {
+struct XInvBody {
   using G = Geo<N>;
   using C = CfgXY<R, N, V>;
-  constexpr int L = C::L, E = C::E;
-  constexpr int F = xmode_fields(MODE);
-  __shared__ cpx<R> buf[N * L];
-  const int iz0 = blockIdx.x * L, iy = blockIdx.y;
-  const int tid = threadIdx.x;
-  const size_t cs = size_t(G::K) * G::K * G::KZ;
-  const R scale = R(1) / (R(N) * R(N) * R(N));
-  const R ky = ks * R(G::kint(iy));
-  __shared__ cpx<R> tws[N];
-  stage_twiddles<R, N>(tws, tw);
-  for (int f = 0; f < F; ++f) {
-    if (f) __syncthreads();  // the previous field's last LDS reads are done
-    auto ld = [&](int n, int ll) -> cpx<R> {
-      // branch-free: clamp to a valid address, load, then select zero
-      const int iz0c = iz0 + ll;
-      const bool ok = iz0c < G::KZ && G::kept(n);
-      const int iz = ok ? iz0c : 0;
-      const int ix = ok ? G::compact_of(n) : 0;
-      const size_t pt = (size_t(ix) * G::K + iy) * G::KZ + iz;
-      const cpx<R> v = cscale(xfield<R, N, MODE>(f, su, sb, pt, cs, ks * R(G::kint(ix)), ky, ks * R(iz)), scale);
-      return ok ? v : cpx<R>{R(0), R(0)};
-    };
-    auto st = [&](int n, int ll, cpx<R> v) {
-      const int iz = iz0 + ll;
-      if (iz < G::KZ) out[((size_t(f) * N + n) * G::K + iy) * G::KZ + iz] = v;
-    };
-    fft_io<R, N, E, +1, 1, true>(buf, L, L, tws, ld, st);
+  static constexpr int SMEM = N * C::L + N;  // line tile + twiddles
+  // Block (bx, by) = (kz tile, iy).
+  __device__ static void run(cpx<R>* sm, int bx, int by, const cpx<R>* __restrict__ su,
+                             const cpx<R>* __restrict__ sb, cpx<R>* __restrict__ out,
+                             const cpx<R>* __restrict__ tw, R ks) {
+    constexpr int L = C::L, E = C::E;
+    constexpr int F = xmode_fields(MODE);
+    cpx<R>* buf = sm;
+    cpx<R>* tws = sm + N * L;
+    const int iz0 = bx * L, iy = by;
+    const size_t cs = size_t(G::K) * G::K * G::KZ;
+    const R scale = R(1) / (R(N) * R(N) * R(N));
+    const R ky = ks * R(G::kint(iy));
+    stage_twiddles<R, N>(tws, tw);
+    for (int f = 0; f < F; ++f) {
+      if (f) __syncthreads();  // the previous field's last LDS reads are done
+      auto ld = [&](int n, int ll) -> cpx<R> {
+        // branch-free: clamp to a valid address, load, then select zero
+        const int iz0c = iz0 + ll;
+        const bool ok = iz0c < G::KZ && G::kept(n);
+        const int iz = ok ? iz0c : 0;
+        const int ix = ok ? G::compact_of(n) : 0;
+        const size_t pt = (size_t(ix) * G::K + iy) * G::KZ + iz;
+        const cpx<R> v = cscale(xfield<R, N, MODE>(f, su, sb, pt, cs, ks * R(G::kint(ix)), ky, ks * R(iz)), scale);
+        return ok ? v : cpx<R>{R(0), R(0)};
+      };
+      auto st = [&](int n, int ll, cpx<R> v) {
+        const int iz = iz0 + ll;
+        if (iz < G::KZ) out[((size_t(f) * N + n) * G::K + iy) * G::KZ + iz] = v;
+      };
+      fft_io<R, N, E, +1, 1, true>(buf, L, L, tws, ld, st);
+    }
   }
+};
+
+template <typename R, int N, int MODE, int V>
+__global__ __launch_bounds__((CfgXY<R, N, V>::NT)) void k_inv_x(const cpx<R>* __restrict__ su,
+                                                              const cpx<R>* __restrict__ sb,
+                                                              cpx<R>* __restrict__ out,
+                                                              const cpx<R>* __restrict__ tw, R ks) {
+  using B = XInvBody<R, N, MODE, V>;
+  __shared__ cpx<R> sm[B::SMEM];
+  B::run(sm, blockIdx.x, blockIdx.y, su, sb, out, tw, ks);
 }
 
 // ----------------------------------------------------------------- pass 2/4
 // Transform along y for F fields.  DIR=+1: A[f][x][iy][iz] (K rows, zero
 // padded) -> B[f][x][y][iz].  DIR=-1: B -> A keeping the K masked rows.
 template <typename R, int N, int DIR, int V>
-__global__ __launch_bounds__((CfgXY<R, N, V>::NT)) void k_y(const cpx<R>* __restrict__ in,
-                                                    cpx<R>* __restrict__ out,
-                                                    const cpx<R>* __restrict__ tw) {
+struct YBody {
   using G = Geo<N>;
   using C = CfgXY<R, N, V>;
-  constexpr int L = C::L, E = C::E;
-  __shared__ cpx<R> buf[N * L];
-  const int iz0 = blockIdx.x * L, x = blockIdx.y, f = blockIdx.z;
-  const int tid = threadIdx.x;
-  const size_t plane = size_t(f) * N + x;
-  __shared__ cpx<R> tws[N];
-  stage_twiddles<R, N>(tws, tw);
-  auto ld = [&](int n, int ll) -> cpx<R> {
-    const int izr = iz0 + ll;
-    bool ok = izr < G::KZ;
-    const int iz = ok ? izr : 0;
-    cpx<R> v;
-    if constexpr (DIR > 0) {
-      ok = ok && G::kept(n);
-      v = in[(plane * G::K + (ok ? G::compact_of(n) : 0)) * G::KZ + iz];
-    } else {
-      v = in[(plane * N + n) * G::KZ + iz];
-    }
-    return ok ? v : cpx<R>{R(0), R(0)};
-  };
-  auto st = [&](int n, int ll, cpx<R> v) {
-    const int iz = iz0 + ll;
-    if (iz >= G::KZ) return;
-    if constexpr (DIR > 0) {
-      out[(plane * N + n) * G::KZ + iz] = v;
-    } else {
-      if (G::kept(n)) out[(plane * G::K + G::compact_of(n)) * G::KZ + iz] = v;
-    }
-  };
-  fft_io<R, N, E, DIR, 1, true>(buf, L, L, tws, ld, st);
+  static constexpr int SMEM = N * C::L + N;
+  // Block (bx, by, bz) = (kz tile, x, field).
+  __device__ static void run(cpx<R>* sm, int bx, int by, int bz, const cpx<R>* __restrict__ in,
+                             cpx<R>* __restrict__ out, const cpx<R>* __restrict__ tw) {
+    constexpr int L = C::L, E = C::E;
+    cpx<R>* buf = sm;
+    cpx<R>* tws = sm + N * L;
+    const int iz0 = bx * L;
+    const size_t plane = size_t(bz) * N + by;
+    stage_twiddles<R, N>(tws, tw);
+    auto ld = [&](int n, int ll) -> cpx<R> {
+      const int izr = iz0 + ll;
+      bool ok = izr < G::KZ;
+      const int iz = ok ? izr : 0;
+      cpx<R> v;
+      if constexpr (DIR > 0) {
+        ok = ok && G::kept(n);
+        v = in[(plane * G::K + (ok ? G::compact_of(n) : 0)) * G::KZ + iz];
+      } else {
+        v = in[(plane * N + n) * G::KZ + iz];
+      }
+      return ok ? v : cpx<R>{R(0), R(0)};
+    };
+    auto st = [&](int n, int ll, cpx<R> v) {
+      const int iz = iz0 + ll;
+      if (iz >= G::KZ) return;
+      if constexpr (DIR > 0) {
+        out[(plane * N + n) * G::KZ + iz] = v;
+      } else {
+        if (G::kept(n)) out[(plane * G::K + G::compact_of(n)) * G::KZ + iz] = v;
+      }
+    };
+    fft_io<R, N, E, DIR, 1, true>(buf, L, L, tws, ld, st);
+  }
+};
+
+template <typename R, int N, int DIR, int V>
+__global__ __launch_bounds__((CfgXY<R, N, V>::NT)) void k_y(const cpx<R>* __restrict__ in,
+                                                          cpx<R>* __restrict__ out,
+                                                          const cpx<R>* __restrict__ tw) {
+  using B = YBody<R, N, DIR, V>;
+  __shared__ cpx<R> sm[B::SMEM];
+  B::run(sm, blockIdx.x, blockIdx.y, blockIdx.z, in, out, tw);
 }
 
 // ----------------------------------------------------------------- pass 3
@@ -209,29 +232,33 @@ __device__ __forceinline__ void cross3(const R* a, const R* b, R* o) {
 }
 
 template <typename R, int N, int MODE, int V>
-__global__ __launch_bounds__((CfgZM<R, N, MODE, V>::NT)) void k_z(const cpx<R>* __restrict__ in,
-                                                   cpx<R>* __restrict__ out,
-                                                   const R* __restrict__ phys_in,
-                                                   R* __restrict__ phys_out,
-                                                   const cpx<R>* __restrict__ tw, R lorentz) {
+struct ZBody {
   using G = Geo<N>;
   using Z = CfgZM<R, N, MODE, V>;
   using S = ZShape<MODE>;
+  static constexpr int PSI = pad_shift<N, Z::E>();
+  static constexpr int PSF = (S::FOUT > 0) ? pad_shift<N, Z::E / 2>() : PSI;
+  static constexpr int PSMIN = PSI < PSF ? PSI : PSF;
+  static constexpr int BUF = N * Z::LP + (N >> PSMIN) + 1;
+  static constexpr int SMEM = BUF + N;
+  // Block (bx, by) = (group of LZ y lines, x).  Threads at or beyond Z::NT
+  // (a batched launch with a larger block) only join the barriers.
+  __device__ static void run(cpx<R>* sm, int bx, int by, const cpx<R>* __restrict__ in,
+                             cpx<R>* __restrict__ out, const R* __restrict__ phys_in,
+                             R* __restrict__ phys_out, const cpx<R>* __restrict__ tw, R lorentz) {
   constexpr int NSEQ = Z::NSEQ, LP = Z::LP, NT = Z::NT, E = Z::E;
   constexpr int LZ = Z::LZ;  // lines per block
   constexpr int KZ = G::KZ;
   static_assert(N % LZ == 0, "lines per block must divide N");
   // padded LDS layouts of the inverse output and of the forward input
-  constexpr int PSI = pad_shift<N, E>();
-  constexpr int PSF = (S::FOUT > 0) ? pad_shift<N, E / 2>() : PSI;
-  constexpr int PSMIN = PSI < PSF ? PSI : PSF;
-  __shared__ cpx<R> buf[N * LP + (N >> PSMIN) + 1];
+  cpx<R>* buf = sm;
+  cpx<R>* tws = sm + BUF;
   auto bi = [&](int n, int l) -> cpx<R>& { return buf[lidx<PSI>(n, l, LP)]; };
-  __shared__ cpx<R> tws[N];
   stage_twiddles<R, N>(tws, tw);
   auto bf = [&](int n, int l) -> cpx<R>& { return buf[lidx<PSF>(n, l, LP)]; };
   const int tid = threadIdx.x;
-  const int x = blockIdx.y, y0 = blockIdx.x * LZ;
+  const bool live = tid < NT;
+  const int x = by, y0 = bx * LZ;
   const size_t NNKZ = size_t(N) * N * KZ;
   const size_t line0 = size_t(x) * N + y0;  // (x, y0) line index
 
@@ -262,14 +289,14 @@ __global__ __launch_bounds__((CfgZM<R, N, MODE, V>::NT)) void k_z(const cpx<R>* 
   }
 
   if constexpr (MODE == ZM_PHYS_OUT3) {
-    for (int idx = tid; idx < LZ * 3 * N; idx += NT) {
+    for (int idx = tid; live && idx < LZ * 3 * N; idx += NT) {
       const int z = idx % N, f = (idx / N) % 3, line = idx / (3 * N);
       const cpx<R> v = bi(z, line * S::SPL + f / 2);
       phys_out[(size_t(f) * N * N + line0 + line) * N + z] = (f & 1) ? v.y : v.x;
     }
     return;
   } else if constexpr (MODE == ZM_PHYS_OUT1) {
-    for (int idx = tid; idx < LZ * N; idx += NT) {
+    for (int idx = tid; live && idx < LZ * N; idx += NT) {
       const int z = idx % N, line = idx / N;
       phys_out[(line0 + line) * N + z] = bi(z, line).x;
     }
@@ -277,16 +304,16 @@ __global__ __launch_bounds__((CfgZM<R, N, MODE, V>::NT)) void k_z(const cpx<R>* 
   } else if constexpr (MODE == ZM_MAXABS) {
     __shared__ R red[NT];
     R m = R(0);
-    for (int idx = tid; idx < LZ * N; idx += NT) {
+    for (int idx = tid; live && idx < LZ * N; idx += NT) {
       const int z = idx % N, line = idx / N;
       m = fmax(m, fabs(bi(z, line).x));
     }
-    red[tid] = m;
+    if (live) red[tid] = m;
     __syncthreads();
     if (tid == 0) {
       R mm = R(0);
       for (int i = 0; i < NT; ++i) mm = fmax(mm, red[i]);
-      phys_out[size_t(blockIdx.y) * gridDim.x + blockIdx.x] = mm;
+      phys_out[size_t(by) * (N / LZ) + bx] = mm;
     }
     return;
   } else {
@@ -301,7 +328,7 @@ __global__ __launch_bounds__((CfgZM<R, N, MODE, V>::NT)) void k_z(const cpx<R>* 
 #pragma unroll
     for (int it = 0; it < IPT; ++it) {
       const int item = tid + it * NT;
-      if (item >= ITEMS) break;
+      if (!live || item >= ITEMS) break;
       const int z = item % N, g = item / N;
       if constexpr (MODE == ZM_HYDRO) {
 #pragma unroll
@@ -344,7 +371,7 @@ __global__ __launch_bounds__((CfgZM<R, N, MODE, V>::NT)) void k_z(const cpx<R>* 
 #pragma unroll
     for (int it = 0; it < IPT; ++it) {
       const int item = tid + it * NT;
-      if (item >= ITEMS) break;
+      if (!live || item >= ITEMS) break;
       const int z = item % N, g = item / N;
       const R* r = res[it];
       if constexpr (MODE == ZM_MHD) {
@@ -361,7 +388,7 @@ __global__ __launch_bounds__((CfgZM<R, N, MODE, V>::NT)) void k_z(const cpx<R>* 
     fft_io<R, N, E / 2, -1, 0, false, true, PSF>(buf, LP, NSEQ / 2, tws, NoIO{}, NoIO{});
     // ---- unpack the kept kz of each pair and store
     constexpr int NQ = NSEQ / 2;
-    for (int idx = tid; idx < NQ * KZ; idx += NT) {
+    for (int idx = tid; live && idx < NQ * KZ; idx += NT) {
       const int kz = idx % KZ, q = idx / KZ;
       const cpx<R> P = bf(kz, q);
       const cpx<R> Qc = bf((N - kz) % N, q);  // conj taken below
@@ -380,6 +407,18 @@ __global__ __launch_bounds__((CfgZM<R, N, MODE, V>::NT)) void k_z(const cpx<R>* 
       out[fb * NNKZ + (line0 + line_b) * KZ + kz] = im;
     }
   }
+  }
+};
+
+template <typename R, int N, int MODE, int V>
+__global__ __launch_bounds__((CfgZM<R, N, MODE, V>::NT)) void k_z(const cpx<R>* __restrict__ in,
+                                                                 cpx<R>* __restrict__ out,
+                                                                 const R* __restrict__ phys_in,
+                                                                 R* __restrict__ phys_out,
+                                                                 const cpx<R>* __restrict__ tw, R lorentz) {
+  using B = ZBody<R, N, MODE, V>;
+  __shared__ cpx<R> sm[B::SMEM];
+  B::run(sm, blockIdx.x, blockIdx.y, in, out, phys_in, phys_out, tw, lorentz);
 }
 
 // ----------------------------------------------------------------- pass 5
@@ -435,18 +474,21 @@ __device__ __forceinline__ void rk4_update(int stage, R dt, size_t pt, size_t cs
 }
 
 template <typename R, int N, int MODE, int V>
-__global__ __launch_bounds__((CfgXF<R, N, V>::NT)) void k_fwd_x(const cpx<R>* __restrict__ in,
-                                                        StageArgs<R> a,
-                                                        const cpx<R>* __restrict__ tw) {
+struct XFBody {
   using G = Geo<N>;
+  static constexpr int LINES = CfgXF<R, N, V>::LINES;
+  static constexpr int SMEM = N * LINES + N;
+  // Block (bx, by) = (kz tile, iy).
+  __device__ static void run(cpx<R>* sm, int bx, int by, const cpx<R>* __restrict__ in, const StageArgs<R>& a,
+                             const cpx<R>* __restrict__ tw) {
   constexpr int FF = (MODE == FM_MHD_STAGE || MODE == FM_MHD_RHS) ? 6 : 3;
-  constexpr int LINES = CfgXF<R, N, V>::LINES, NT = CfgXF<R, N, V>::NT, E = CfgXF<R, N, V>::E;
+  constexpr int NT = CfgXF<R, N, V>::NT, E = CfgXF<R, N, V>::E;
   constexpr int L = LINES / FF;
   constexpr int LP = LINES;
-  __shared__ cpx<R> buf[N * LP];
-  const int iz0 = blockIdx.x * L, iy = blockIdx.y;
+  cpx<R>* buf = sm;
+  cpx<R>* tws = sm + N * LP;
+  const int iz0 = bx * L, iy = by;
   const int tid = threadIdx.x;
-  __shared__ cpx<R> tws[N];
   stage_twiddles<R, N>(tws, tw);
   auto ld = [&](int n, int ln) -> cpx<R> {
     const int f = ln / L, izr = iz0 + ln % L;
@@ -457,7 +499,7 @@ __global__ __launch_bounds__((CfgXF<R, N, V>::NT)) void k_fwd_x(const cpx<R>* __
   fft_io<R, N, E, -1, 1, false>(buf, LP, LINES, tws, ld, NoIO{});
   const size_t cs = size_t(G::K) * G::K * G::KZ;
   const R ky = a.ks * R(G::kint(iy));
-  for (int idx = tid; idx < G::K * L; idx += NT) {
+  for (int idx = tid; tid < NT && idx < G::K * L; idx += NT) {
     const int ll = idx % L, ix = idx / L, iz = iz0 + ll;
     if (iz >= G::KZ) continue;
     const int n = G::full_of(ix);
@@ -514,6 +556,16 @@ __global__ __launch_bounds__((CfgXF<R, N, V>::NT)) void k_fwd_x(const cpx<R>* __
       }
     }
   }
+  }
+};
+
+template <typename R, int N, int MODE, int V>
+__global__ __launch_bounds__((CfgXF<R, N, V>::NT)) void k_fwd_x(const cpx<R>* __restrict__ in,
+                                                              StageArgs<R> a,
+                                                              const cpx<R>* __restrict__ tw) {
+  using B = XFBody<R, N, MODE, V>;
+  __shared__ cpx<R> sm[B::SMEM];
+  B::run(sm, blockIdx.x, blockIdx.y, in, a, tw);
 }
 
 // ----------------------------------------------------------------- diagnostics

@@ -11,6 +11,7 @@
 
 #include "../../include/tgdns.h"
 #include "tg_passes.hpp"
+#include "tg_batch.hpp"
 
 using namespace tg;
 
@@ -585,6 +586,234 @@ int tg_timing_read(tg_plan* p, double* ms, long long* count) {
     count[p->evpass[i]] += 1;
   }
   p->evused = 0;
+  return TG_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- ensembles
+struct tg_batch {
+  int ninst = 0, prec = 64;
+  double ks = 1.0;
+  std::vector<int> n;
+  std::vector<double> nu;
+  std::vector<long long> state_off;  // complex elements, ninst + 1 entries
+  std::vector<size_t> ws_off;        // bytes, ninst + 1 entries
+  std::vector<int> prefix;           // EP_COUNT x (ninst + 1)
+  std::vector<void*> tw_by_size;     // twiddles per entry of kEnsSizes
+  void* d_inst = nullptr;            // EnsInst<R>[ninst]
+  int* d_prefix = nullptr;
+  size_t cb = 16;
+  bool bound = false;
+  hipStream_t cap = nullptr;         // private capture stream
+  hipGraphExec_t exec = nullptr;
+  double graph_dt = 0.0;
+};
+
+namespace {
+
+int ens_size_index(int n) {
+  for (int i = 0; i < kNumEnsSizes; ++i)
+    if (kEnsSizes[i] == n) return i;
+  return -1;
+}
+
+template <typename R>
+int batch_launch_step(tg_batch* b, double dt, hipStream_t s) {
+  const auto* inst = static_cast<const EnsInst<R>*>(b->d_inst);
+  for (int st = 0; st < 4; ++st) {
+    for (int pass = 0; pass < EP_COUNT; ++pass) {
+      const int nblk = b->prefix[pass * (b->ninst + 1) + b->ninst];
+      const int* pre = b->d_prefix + pass * (b->ninst + 1);
+      switch (pass) {
+        case EP_INV_X:
+          hipLaunchKernelGGL((kb_pass<R, EP_INV_X>), dim3(nblk), dim3(EnsShape<R, EP_INV_X>::NT), 0, s, inst, pre,
+                             b->ninst, R(dt), st);
+          break;
+        case EP_INV_Y:
+          hipLaunchKernelGGL((kb_pass<R, EP_INV_Y>), dim3(nblk), dim3(EnsShape<R, EP_INV_Y>::NT), 0, s, inst, pre,
+                             b->ninst, R(dt), st);
+          break;
+        case EP_Z:
+          hipLaunchKernelGGL((kb_pass<R, EP_Z>), dim3(nblk), dim3(EnsShape<R, EP_Z>::NT), 0, s, inst, pre,
+                             b->ninst, R(dt), st);
+          break;
+        case EP_FWD_Y:
+          hipLaunchKernelGGL((kb_pass<R, EP_FWD_Y>), dim3(nblk), dim3(EnsShape<R, EP_FWD_Y>::NT), 0, s, inst, pre,
+                             b->ninst, R(dt), st);
+          break;
+        default:
+          hipLaunchKernelGGL((kb_pass<R, EP_FWD_X>), dim3(nblk), dim3(EnsShape<R, EP_FWD_X>::NT), 0, s, inst, pre,
+                             b->ninst, R(dt), st);
+          break;
+      }
+      TG_LAUNCHED();
+    }
+  }
+  return TG_OK;
+}
+
+template <typename R>
+int batch_bind(tg_batch* b, void* state, void* ws) {
+  std::vector<EnsInst<R>> h(b->ninst);
+  char* w = static_cast<char*>(ws);
+  cpx<R>* st = static_cast<cpx<R>*>(state);
+  for (int i = 0; i < b->ninst; ++i) {
+    const int N = b->n[i], M = N / 3, K = 2 * M + 1, KZ = M + 1;
+    const size_t vec = size_t(3) * K * K * KZ * sizeof(cpx<R>);
+    const size_t sA = size_t(6) * N * K * KZ * sizeof(cpx<R>);
+    char* base = w + b->ws_off[i];
+    EnsInst<R>& I = h[i];
+    I.n = N;
+    I.nu = R(b->nu[i]);
+    I.ks = R(b->ks);
+    I.u0 = st + b->state_off[i];
+    I.u1 = reinterpret_cast<cpx<R>*>(base);
+    I.acc = reinterpret_cast<cpx<R>*>(base + align256(vec));
+    I.A = reinterpret_cast<cpx<R>*>(base + 2 * align256(vec));
+    I.B = reinterpret_cast<cpx<R>*>(base + 2 * align256(vec) + align256(sA));
+    I.tw = static_cast<const cpx<R>*>(b->tw_by_size[ens_size_index(N)]);
+  }
+  TG_HIP(hipMemcpy(b->d_inst, h.data(), h.size() * sizeof(EnsInst<R>), hipMemcpyHostToDevice));
+  b->bound = true;
+  return TG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tg_batch_create(int ninst, const int* n, const double* nu, int precision_bits, double domain_length,
+                    tg_batch** out) {
+  if (!out) return fail(TG_ERR_ARG, "out is NULL");
+  *out = nullptr;
+  if (ninst <= 0 || !n || !nu) return fail(TG_ERR_ARG, "tg_batch_create: need ninst > 0, n and nu");
+  if (precision_bits != 64 && precision_bits != 32) return fail(TG_ERR_ARG, "precision_bits must be 64 or 32");
+  if (!(domain_length > 0.0)) return fail(TG_ERR_ARG, "domain_length must be positive");
+  for (int i = 0; i < ninst; ++i) {
+    if (ens_size_index(n[i]) < 0)
+      return fail(TG_ERR_UNSUPPORTED, "ensemble grid size N=" + std::to_string(n[i]) + " has no batched plan");
+    if (!(nu[i] >= 0.0)) return fail(TG_ERR_ARG, "viscosity must be nonnegative");
+  }
+  tg_batch* b = new tg_batch{};
+  b->ninst = ninst;
+  b->prec = precision_bits;
+  b->cb = precision_bits == 64 ? 16 : 8;
+  b->ks = domain_length == 2.0 * M_PI ? 1.0 : 2.0 * M_PI / domain_length;
+  b->n.assign(n, n + ninst);
+  b->nu.assign(nu, nu + ninst);
+  b->state_off.assign(ninst + 1, 0);
+  b->ws_off.assign(ninst + 1, 0);
+  for (int i = 0; i < ninst; ++i) {
+    const size_t N = n[i], M = N / 3, K = 2 * M + 1, KZ = M + 1;
+    const size_t vec = 3 * K * K * KZ * b->cb;
+    b->state_off[i + 1] = b->state_off[i] + (long long)(3 * K * K * KZ);
+    b->ws_off[i + 1] = b->ws_off[i] + 2 * align256(vec) + align256(6 * N * K * KZ * b->cb) +
+                       align256(6 * N * N * KZ * b->cb);
+  }
+  b->prefix.assign(EP_COUNT * (ninst + 1), 0);
+  for (int pass = 0; pass < EP_COUNT; ++pass)
+    for (int i = 0; i < ninst; ++i)
+      b->prefix[pass * (ninst + 1) + i + 1] =
+          b->prefix[pass * (ninst + 1) + i] +
+          (precision_bits == 64 ? ens_blocks<double>(n[i], pass) : ens_blocks<float>(n[i], pass));
+  auto cleanup = [&](hipError_t e, const char* what) {
+    for (void* t : b->tw_by_size)
+      if (t) (void)hipFree(t);
+    if (b->d_inst) (void)hipFree(b->d_inst);
+    if (b->d_prefix) (void)hipFree(b->d_prefix);
+    delete b;
+    return fail(TG_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+  };
+  b->tw_by_size.assign(kNumEnsSizes, nullptr);
+  for (int k = 0; k < kNumEnsSizes; ++k) {
+    const size_t N = kEnsSizes[k];
+    std::vector<char> host(N * b->cb);
+    for (size_t j = 0; j < N; ++j) {
+      const long double ang = -2.0L * 3.14159265358979323846264338327950288L * (long double)j / (long double)N;
+      if (b->prec == 64) {
+        double v[2] = {double(cosl(ang)), double(sinl(ang))};
+        memcpy(host.data() + j * 16, v, 16);
+      } else {
+        float v[2] = {float(cosl(ang)), float(sinl(ang))};
+        memcpy(host.data() + j * 8, v, 8);
+      }
+    }
+    hipError_t e = hipMalloc(&b->tw_by_size[k], host.size());
+    if (e != hipSuccess) return cleanup(e, "hipMalloc(twiddles)");
+    e = hipMemcpy(b->tw_by_size[k], host.data(), host.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return cleanup(e, "hipMemcpy(twiddles)");
+  }
+  const size_t instBytes = size_t(ninst) * (b->prec == 64 ? sizeof(EnsInst<double>) : sizeof(EnsInst<float>));
+  hipError_t e = hipMalloc(&b->d_inst, instBytes);
+  if (e != hipSuccess) return cleanup(e, "hipMalloc(descriptors)");
+  e = hipMalloc(reinterpret_cast<void**>(&b->d_prefix), b->prefix.size() * sizeof(int));
+  if (e != hipSuccess) return cleanup(e, "hipMalloc(prefix)");
+  e = hipMemcpy(b->d_prefix, b->prefix.data(), b->prefix.size() * sizeof(int), hipMemcpyHostToDevice);
+  if (e != hipSuccess) return cleanup(e, "hipMemcpy(prefix)");
+  *out = b;
+  return TG_OK;
+}
+
+int tg_batch_destroy(tg_batch* b) {
+  if (!b) return TG_OK;
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->cap) (void)hipStreamDestroy(b->cap);
+  for (void* t : b->tw_by_size)
+    if (t) (void)hipFree(t);
+  if (b->d_inst) (void)hipFree(b->d_inst);
+  if (b->d_prefix) (void)hipFree(b->d_prefix);
+  delete b;
+  return TG_OK;
+}
+
+int tg_batch_query(const tg_batch* b, long long* state_offsets, long long* ws_bytes) {
+  if (!b) return fail(TG_ERR_ARG, "batch is NULL");
+  if (state_offsets)
+    for (int i = 0; i <= b->ninst; ++i) state_offsets[i] = b->state_off[i];
+  if (ws_bytes) *ws_bytes = (long long)b->ws_off[b->ninst];
+  return TG_OK;
+}
+
+int tg_batch_bind(tg_batch* b, void* state, void* ws) {
+  if (!b) return fail(TG_ERR_ARG, "batch is NULL");
+  if (!state || !ws) return fail(TG_ERR_ARG, "tg_batch_bind: NULL buffer");
+  return b->prec == 64 ? batch_bind<double>(b, state, ws) : batch_bind<float>(b, state, ws);
+}
+
+int tg_batch_rk4_steps(tg_batch* b, int nsteps, double dt, int use_graph, void* stream) {
+  if (!b) return fail(TG_ERR_ARG, "batch is NULL");
+  if (!b->bound) return fail(TG_ERR_ARG, "tg_batch_rk4_steps: call tg_batch_bind first");
+  if (nsteps < 0) return fail(TG_ERR_ARG, "nsteps must be >= 0");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  auto step = [&](hipStream_t st) {
+    return b->prec == 64 ? batch_launch_step<double>(b, dt, st) : batch_launch_step<float>(b, dt, st);
+  };
+  if (!use_graph) {
+    for (int i = 0; i < nsteps; ++i) TG_RET(step(s));
+    return TG_OK;
+  }
+  if (!b->exec || b->graph_dt != dt) {
+    if (b->exec) {
+      TG_HIP(hipGraphExecDestroy(b->exec));
+      b->exec = nullptr;
+    }
+    if (!b->cap) TG_HIP(hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking));
+    hipGraph_t g = nullptr;
+    TG_HIP(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal));
+    const int rc = step(b->cap);
+    hipError_t e = hipStreamEndCapture(b->cap, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return fail(TG_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiateWithFlags(&b->exec, g, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return fail(TG_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    b->graph_dt = dt;
+  }
+  for (int i = 0; i < nsteps; ++i) TG_HIP(hipGraphLaunch(b->exec, s));
   return TG_OK;
 }
 
