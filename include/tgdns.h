@@ -145,7 +145,7 @@ int tg_timing_read(tg_plan* plan, double* ms, long long* count);
  * instance; each instance is the advance_one_step of
  * jax/taylor_green_jax.py:188-196 on its own grid).  Instances may have
  * different N (16, 32, 48, 64, 96, 128) and viscosities; one launch per pass
- * advances all of them.  Instance i's compact state (3 components, plan
+ * and grid size advances all instances of that size.  Instance i's compact state (3 components, plan
  * layout for its N) lives at complex-element offset state_offsets[i] of one
  * caller-owned state buffer; the workspace is one buffer of ws_bytes. */
 typedef struct tg_batch tg_batch;

@@ -1,13 +1,13 @@
 // Batched ensemble: many independent hydro instances of different grid
-// sizes advance together, one launch per pass for the whole ensemble.
+// sizes advance together, one launch per pass and grid size.
 //
-// A batched launch is a 1D grid whose blocks are the concatenation of every
-// instance's own grid for that pass.  Each block finds its instance with a
-// binary search in a per-pass prefix table, reads the instance descriptor
-// (size, viscosity, buffer pointers) and runs the size-templated pass body
-// (the same code the single-instance kernels run) on its local block index.
-// The block size is the largest any compiled size needs; smaller bodies
-// leave the extra threads idle at the barriers.
+// Instances are grouped by grid size.  For every pass, one launch per size
+// class covers all instances of that size: a 1D grid whose blocks are the
+// concatenation of each instance's own grid.  Each block finds its instance
+// with a binary search in the class's prefix table, reads the instance
+// descriptor (viscosity, buffer pointers) and runs the size-templated pass
+// body (the same code the single-instance kernels run) on its local block
+// index, with that body's exact block shape.
 #pragma once
 #include "tg_passes.hpp"
 
@@ -52,28 +52,6 @@ struct EnsGrid {
   }
 };
 
-template <typename R, int PASS, int... Ns>
-struct EnsMax;
-template <typename R, int PASS, int N0, int... Ns>
-struct EnsMax<R, PASS, N0, Ns...> {
-  static constexpr int nt0 = PASS == EP_Z ? CfgZM<R, N0, ZM_HYDRO, 0>::NT
-                            : PASS == EP_FWD_X ? CfgXF<R, N0, 0>::NT : CfgXY<R, N0, 0>::NT;
-  static constexpr int sm0 = PASS == EP_INV_X ? XInvBody<R, N0, XM_HYDRO, 0>::SMEM
-                            : PASS == EP_Z ? ZBody<R, N0, ZM_HYDRO, 0>::SMEM
-                            : PASS == EP_FWD_X ? XFBody<R, N0, FM_HYDRO_STAGE, 0>::SMEM
-                            : YBody<R, N0, 1, 0>::SMEM;
-  static constexpr int NT = nt0 > EnsMax<R, PASS, Ns...>::NT ? nt0 : EnsMax<R, PASS, Ns...>::NT;
-  static constexpr int SMEM = sm0 > EnsMax<R, PASS, Ns...>::SMEM ? sm0 : EnsMax<R, PASS, Ns...>::SMEM;
-};
-template <typename R, int PASS>
-struct EnsMax<R, PASS> {
-  static constexpr int NT = 64;
-  static constexpr int SMEM = 1;
-};
-
-template <typename R, int PASS>
-using EnsShape = EnsMax<R, PASS, TG_ENS_SIZES_LIST>;
-
 template <typename R, int N, int PASS>
 __device__ __forceinline__ void ens_run(cpx<R>* sm, int lb, const EnsInst<R>& I, R dt, int stage) {
   using Gd = EnsGrid<R, N>;
@@ -101,20 +79,24 @@ __device__ __forceinline__ void ens_run(cpx<R>* sm, int lb, const EnsInst<R>& I,
   }
 }
 
-template <typename R, int PASS, int N0, int... Ns>
-__device__ __forceinline__ void ens_dispatch(int n, cpx<R>* sm, int lb, const EnsInst<R>& I, R dt, int stage) {
-  if (n == N0) {
-    ens_run<R, N0, PASS>(sm, lb, I, dt, stage);
-    return;
-  }
-  if constexpr (sizeof...(Ns) > 0) ens_dispatch<R, PASS, Ns...>(n, sm, lb, I, dt, stage);
-}
+template <typename R, int N, int PASS>
+struct EnsBlock {
+  static constexpr int NT = PASS == EP_Z ? CfgZM<R, N, ZM_HYDRO, 0>::NT
+                          : PASS == EP_FWD_X ? CfgXF<R, N, 0>::NT : CfgXY<R, N, 0>::NT;
+  static constexpr int SMEM = PASS == EP_INV_X ? XInvBody<R, N, XM_HYDRO, 0>::SMEM
+                            : PASS == EP_Z ? ZBody<R, N, ZM_HYDRO, 0>::SMEM
+                            : PASS == EP_FWD_X ? XFBody<R, N, FM_HYDRO_STAGE, 0>::SMEM
+                            : YBody<R, N, 1, 0>::SMEM;
+};
 
-template <typename R, int PASS>
-__global__ __launch_bounds__((EnsShape<R, PASS>::NT)) void kb_pass(const EnsInst<R>* __restrict__ inst,
-                                                                  const int* __restrict__ prefix, int ninst,
-                                                                  R dt, int stage) {
-  __shared__ cpx<R> sm[EnsShape<R, PASS>::SMEM];
+// One launch covers every instance of grid size N (a size class): the
+// descriptors of the class are contiguous, `prefix` holds their block
+// offsets for this pass.
+template <typename R, int N, int PASS>
+__global__ __launch_bounds__((EnsBlock<R, N, PASS>::NT)) void kb_pass(const EnsInst<R>* __restrict__ inst,
+                                                                     const int* __restrict__ prefix, int ninst,
+                                                                     R dt, int stage) {
+  __shared__ cpx<R> sm[EnsBlock<R, N, PASS>::SMEM];
   const int b = blockIdx.x;
   int lo = 0, hi = ninst;  // prefix[lo] <= b < prefix[hi]
   while (hi - lo > 1) {
@@ -122,8 +104,7 @@ __global__ __launch_bounds__((EnsShape<R, PASS>::NT)) void kb_pass(const EnsInst
     if (prefix[mid] <= b) lo = mid;
     else hi = mid;
   }
-  const EnsInst<R> I = inst[lo];
-  ens_dispatch<R, PASS, TG_ENS_SIZES_LIST>(I.n, sm, b - prefix[lo], I, dt, stage);
+  ens_run<R, N, PASS>(sm, b - prefix[lo], inst[lo], dt, stage);
 }
 
 // Host-side block counts (same formulas as EnsGrid).

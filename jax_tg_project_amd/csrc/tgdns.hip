@@ -599,7 +599,10 @@ struct tg_batch {
   std::vector<double> nu;
   std::vector<long long> state_off;  // complex elements, ninst + 1 entries
   std::vector<size_t> ws_off;        // bytes, ninst + 1 entries
-  std::vector<int> prefix;           // EP_COUNT x (ninst + 1)
+  std::vector<int> order;            // descriptor slot -> instance (grouped by size)
+  std::vector<int> cls_n, cls_first, cls_count;  // size classes over descriptor slots
+  std::vector<size_t> cls_prefix_off;            // offset of each class's EP_COUNT x (count+1) table
+  std::vector<int> prefix;
   std::vector<void*> tw_by_size;     // twiddles per entry of kEnsSizes
   void* d_inst = nullptr;            // EnsInst<R>[ninst]
   int* d_prefix = nullptr;
@@ -618,36 +621,59 @@ int ens_size_index(int n) {
   return -1;
 }
 
+template <typename R, int N>
+int batch_launch_class(const tg_batch* b, int c, int st, double dt, hipStream_t s) {
+  const int first = b->cls_first[c], cnt = b->cls_count[c];
+  const auto* inst = static_cast<const EnsInst<R>*>(b->d_inst) + first;
+  for (int pass = 0; pass < EP_COUNT; ++pass) {
+    const size_t po = b->cls_prefix_off[c] + size_t(pass) * (cnt + 1);
+    const int nblk = b->prefix[po + cnt];
+    const int* pre = b->d_prefix + po;
+    switch (pass) {
+      case EP_INV_X:
+        hipLaunchKernelGGL((kb_pass<R, N, EP_INV_X>), dim3(nblk), dim3(EnsBlock<R, N, EP_INV_X>::NT), 0, s, inst,
+                           pre, cnt, R(dt), st);
+        break;
+      case EP_INV_Y:
+        hipLaunchKernelGGL((kb_pass<R, N, EP_INV_Y>), dim3(nblk), dim3(EnsBlock<R, N, EP_INV_Y>::NT), 0, s, inst,
+                           pre, cnt, R(dt), st);
+        break;
+      case EP_Z:
+        hipLaunchKernelGGL((kb_pass<R, N, EP_Z>), dim3(nblk), dim3(EnsBlock<R, N, EP_Z>::NT), 0, s, inst, pre, cnt,
+                           R(dt), st);
+        break;
+      case EP_FWD_Y:
+        hipLaunchKernelGGL((kb_pass<R, N, EP_FWD_Y>), dim3(nblk), dim3(EnsBlock<R, N, EP_FWD_Y>::NT), 0, s, inst,
+                           pre, cnt, R(dt), st);
+        break;
+      default:
+        hipLaunchKernelGGL((kb_pass<R, N, EP_FWD_X>), dim3(nblk), dim3(EnsBlock<R, N, EP_FWD_X>::NT), 0, s, inst,
+                           pre, cnt, R(dt), st);
+        break;
+    }
+    TG_LAUNCHED();
+  }
+  return TG_OK;
+}
+
+// One RK4 step: for each stage, the five passes of every size class.  Size
+// classes are independent, so their launches for one pass are issued back to
+// back before the next pass.
 template <typename R>
 int batch_launch_step(tg_batch* b, double dt, hipStream_t s) {
-  const auto* inst = static_cast<const EnsInst<R>*>(b->d_inst);
   for (int st = 0; st < 4; ++st) {
-    for (int pass = 0; pass < EP_COUNT; ++pass) {
-      const int nblk = b->prefix[pass * (b->ninst + 1) + b->ninst];
-      const int* pre = b->d_prefix + pass * (b->ninst + 1);
-      switch (pass) {
-        case EP_INV_X:
-          hipLaunchKernelGGL((kb_pass<R, EP_INV_X>), dim3(nblk), dim3(EnsShape<R, EP_INV_X>::NT), 0, s, inst, pre,
-                             b->ninst, R(dt), st);
-          break;
-        case EP_INV_Y:
-          hipLaunchKernelGGL((kb_pass<R, EP_INV_Y>), dim3(nblk), dim3(EnsShape<R, EP_INV_Y>::NT), 0, s, inst, pre,
-                             b->ninst, R(dt), st);
-          break;
-        case EP_Z:
-          hipLaunchKernelGGL((kb_pass<R, EP_Z>), dim3(nblk), dim3(EnsShape<R, EP_Z>::NT), 0, s, inst, pre,
-                             b->ninst, R(dt), st);
-          break;
-        case EP_FWD_Y:
-          hipLaunchKernelGGL((kb_pass<R, EP_FWD_Y>), dim3(nblk), dim3(EnsShape<R, EP_FWD_Y>::NT), 0, s, inst, pre,
-                             b->ninst, R(dt), st);
-          break;
-        default:
-          hipLaunchKernelGGL((kb_pass<R, EP_FWD_X>), dim3(nblk), dim3(EnsShape<R, EP_FWD_X>::NT), 0, s, inst, pre,
-                             b->ninst, R(dt), st);
-          break;
+    for (size_t c = 0; c < b->cls_n.size(); ++c) {
+      int rc = TG_OK;
+      switch (b->cls_n[c]) {
+        case 16: rc = batch_launch_class<R, 16>(b, int(c), st, dt, s); break;
+        case 32: rc = batch_launch_class<R, 32>(b, int(c), st, dt, s); break;
+        case 48: rc = batch_launch_class<R, 48>(b, int(c), st, dt, s); break;
+        case 64: rc = batch_launch_class<R, 64>(b, int(c), st, dt, s); break;
+        case 96: rc = batch_launch_class<R, 96>(b, int(c), st, dt, s); break;
+        case 128: rc = batch_launch_class<R, 128>(b, int(c), st, dt, s); break;
+        default: rc = fail(TG_ERR_UNSUPPORTED, "ensemble size class");
       }
-      TG_LAUNCHED();
+      if (rc) return rc;
     }
   }
   return TG_OK;
@@ -658,12 +684,13 @@ int batch_bind(tg_batch* b, void* state, void* ws) {
   std::vector<EnsInst<R>> h(b->ninst);
   char* w = static_cast<char*>(ws);
   cpx<R>* st = static_cast<cpx<R>*>(state);
-  for (int i = 0; i < b->ninst; ++i) {
+  for (int slot = 0; slot < b->ninst; ++slot) {
+    const int i = b->order[slot];
     const int N = b->n[i], M = N / 3, K = 2 * M + 1, KZ = M + 1;
     const size_t vec = size_t(3) * K * K * KZ * sizeof(cpx<R>);
     const size_t sA = size_t(6) * N * K * KZ * sizeof(cpx<R>);
     char* base = w + b->ws_off[i];
-    EnsInst<R>& I = h[i];
+    EnsInst<R>& I = h[slot];
     I.n = N;
     I.nu = R(b->nu[i]);
     I.ks = R(b->ks);
@@ -711,12 +738,28 @@ int tg_batch_create(int ninst, const int* n, const double* nu, int precision_bit
     b->ws_off[i + 1] = b->ws_off[i] + 2 * align256(vec) + align256(6 * N * K * KZ * b->cb) +
                        align256(6 * N * N * KZ * b->cb);
   }
-  b->prefix.assign(EP_COUNT * (ninst + 1), 0);
-  for (int pass = 0; pass < EP_COUNT; ++pass)
+  for (int k = 0; k < kNumEnsSizes; ++k) {
+    int cnt = 0;
     for (int i = 0; i < ninst; ++i)
-      b->prefix[pass * (ninst + 1) + i + 1] =
-          b->prefix[pass * (ninst + 1) + i] +
-          (precision_bits == 64 ? ens_blocks<double>(n[i], pass) : ens_blocks<float>(n[i], pass));
+      if (n[i] == kEnsSizes[k]) {
+        if (!cnt) {
+          b->cls_n.push_back(kEnsSizes[k]);
+          b->cls_first.push_back(int(b->order.size()));
+        }
+        b->order.push_back(i);
+        ++cnt;
+      }
+    if (cnt) b->cls_count.push_back(cnt);
+  }
+  for (size_t c = 0; c < b->cls_n.size(); ++c) {
+    const int cnt = b->cls_count[c];
+    b->cls_prefix_off.push_back(b->prefix.size());
+    for (int pass = 0; pass < EP_COUNT; ++pass) {
+      const int per = precision_bits == 64 ? ens_blocks<double>(b->cls_n[c], pass)
+                                           : ens_blocks<float>(b->cls_n[c], pass);
+      for (int j = 0; j <= cnt; ++j) b->prefix.push_back(j * per);
+    }
+  }
   auto cleanup = [&](hipError_t e, const char* what) {
     for (void* t : b->tw_by_size)
       if (t) (void)hipFree(t);
