@@ -128,6 +128,36 @@ def run_ensemble(ninst, steps, dt, seed=0):
             "graph_equals_eager": same, "hbm_GBps_algorithmic": nbytes / wall / 1e9}
 
 
+def run_adjoint(N, steps, dt, nu, dtype):
+    """Adjoint timing at N with J = Z(T): forward = `steps` plain RK4 steps;
+    adjoint = forward sweep with sqrt(n) checkpoints + reverse sweep."""
+    import torch
+    from jax_tg_project_amd._lib import Plan
+    from jax_tg_project_amd import ensemble
+    from jax_tg_project_amd.adjoint import sqrt_segment
+    plan = Plan(N, dtype=dtype)
+    x0 = plan.forward(torch.tensor(ensemble.perturbed_taylor_green(N, 2024), dtype=dtype, device=plan.device))
+    seg = sqrt_segment(steps)
+    aws = plan.adjoint_workspace(steps, seg)
+    y = x0.clone()
+    plan.rk4(y, nsteps=2, dt=dt, nu=nu)  # warm-up
+    J = plan.adjoint_forward(x0, 2, dt, nu, 1, 1, plan.adjoint_workspace(2, 1))
+    torch.cuda.synchronize()
+    y = x0.clone()
+    t0 = time.perf_counter()
+    plan.rk4(y, nsteps=steps, dt=dt, nu=nu)
+    torch.cuda.synchronize()
+    fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    J = plan.adjoint_forward(x0, steps, dt, nu, 1, seg, aws)
+    g, dnu = plan.adjoint_backward(steps, dt, nu, seg, aws)
+    torch.cuda.synchronize()
+    adj = time.perf_counter() - t0
+    return {"N": N, "steps": steps, "segment": seg, "dtype": "f64" if dtype == torch.float64 else "f32",
+            "forward_s": fwd, "forward_plus_adjoint_s": adj, "adjoint_over_forward": adj / fwd,
+            "J_enstrophy": J, "dJ_dnu": dnu}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -138,6 +168,7 @@ def main():
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this pseudo-spectral path runs on one GPU; --gpus {args.gpus} is not supported "
@@ -164,7 +195,7 @@ def main():
                              / (v[0] / max(1, v[1]) * 1e-3) / 1e9)} for k, v in pas.items()}
 
     res = {
-        "metric": "RK4 steps/s + HBM GB/s at 256^3 (hydro TGV, FP64); ensemble solves/s",
+        "metric": "RK4 steps/s + HBM GB/s at 256^3 (hydro TGV, FP64); adjoint/forward time; ensemble solves/s",
         "value": steps_s,
         "unit": "RK4 steps/s",
         "n_gpus": 1,
@@ -205,6 +236,10 @@ def main():
 
     if not args.no_ensemble:
         res["ensemble"] = run_ensemble(64, 200, 5e-3)
+        torch.cuda.empty_cache()
+
+    if not args.no_adjoint:
+        res["adjoint"] = [run_adjoint(128, 200, 5e-3, 1.0 / 400, dt_) for dt_ in (torch.float64, torch.float32)]
         torch.cuda.empty_cache()
 
     if not args.no_cpu_baseline:

@@ -161,6 +161,30 @@ int tg_batch_bind(tg_batch* batch, void* state, void* ws);
  * into a HIP graph (again when dt changes) and replayed nsteps times. */
 int tg_batch_rk4_steps(tg_batch* batch, int nsteps, double dt, int use_graph, void* stream);
 
+/* ---- Adjoint (no counterpart in the reference, whose only gradient would be
+ * jax.grad through advance_one_step, jax/taylor_green_jax.py:188-196).
+ * Cotangents are unnormalised spectra of physical gradients under the
+ * physical L2 inner product <f, g> = sum over grid points of f . g.
+ * tg_vjp: out = J_rhs(x)^T a = mask F(w x q) + i k x mask F(q x u) - nu k^2 a,
+ *   q = F^-1(P a), u = F^-1 x, w = curl u (hydro plans). */
+int tg_vjp(tg_plan* plan, const void* x_hat, const void* a_hat, double nu, void* out_hat, void* ws, void* stream);
+/* Bytes of adjoint workspace for tg_adjoint_grad(nsteps, seg). */
+int tg_adjoint_workspace(tg_plan* plan, int nsteps, int seg, long long* bytes);
+/* Forward sweep: J = E(T) (objective 0) or Z(T) (1) after nsteps RK4 steps of
+ * size dt from x0_hat with viscosity nu, keeping a checkpoint every `seg`
+ * steps (seg = ceil(sqrt(nsteps)) is the sqrt-n schedule) in adj_ws.
+ * Synchronises `stream` (J is returned to the host). */
+int tg_adjoint_forward(tg_plan* plan, const void* x0_hat, int nsteps, double dt, double nu, int objective, int seg,
+                       double* J, void* ws, void* adj_ws, void* stream);
+/* Reverse sweep after tg_adjoint_forward with the same nsteps/dt/nu/seg and
+ * adj_ws: each segment's steps are recomputed from its checkpoint once,
+ * keeping their stage states, then reversed with four vector-Jacobian
+ * products per step.  grad_hat receives dJ/d(stored entries of x0_hat) as
+ * d/dRe + i d/dIm (the convention of torch's complex .grad); *dJdnu the
+ * viscosity derivative.  Synchronises `stream`. */
+int tg_adjoint_backward(tg_plan* plan, int nsteps, double dt, double nu, int seg, void* grad_hat, double* dJdnu,
+                        void* ws, void* adj_ws, void* stream);
+
 /* Copy the calling thread's last error message into buf (NUL-terminated). */
 int tg_last_error(char* buf, size_t n);
 

@@ -48,6 +48,12 @@ _SIGS = {
                        ctypes.c_int),
     "tg_batch_bind": ([_VP, _VP, _VP], ctypes.c_int),
     "tg_batch_rk4_steps": ([_VP, ctypes.c_int, ctypes.c_double, ctypes.c_int, _VP], ctypes.c_int),
+    "tg_vjp": ([_VP, _VP, _VP, ctypes.c_double, _VP, _VP, _VP], ctypes.c_int),
+    "tg_adjoint_workspace": ([_VP, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_longlong)], ctypes.c_int),
+    "tg_adjoint_forward": ([_VP, _VP, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                            ctypes.POINTER(ctypes.c_double), _VP, _VP, _VP], ctypes.c_int),
+    "tg_adjoint_backward": ([_VP, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, _VP,
+                             ctypes.POINTER(ctypes.c_double), _VP, _VP, _VP], ctypes.c_int),
     "tg_last_error": ([ctypes.c_char_p, ctypes.c_size_t], ctypes.c_int),
 }
 PASS_NAMES = ("inv_x", "inv_y", "z", "fwd_y", "fwd_x")
@@ -214,6 +220,40 @@ class Plan:
             _check(self._lib.tg_rk4_steps(self._h, _ptr(u_hat), _ptr(b_hat), int(nsteps), float(dt),
                                           float(nu), float(eta), float(lorentz_sign), _ptr(self.ws),
                                           _stream()))
+
+    # -- adjoint ---------------------------------------------------------------
+    def vjp(self, x_hat, a_hat, nu):
+        """J_rhs(x)^T a (physical-L2 cotangents as unnormalised spectra)."""
+        self._spec(x_hat, "x_hat")
+        self._spec(a_hat, "a_hat")
+        out = self.empty_spec()
+        with torch.cuda.device(self.device):
+            _check(self._lib.tg_vjp(self._h, _ptr(x_hat), _ptr(a_hat), float(nu), _ptr(out), _ptr(self.ws),
+                                    _stream()))
+        return out
+
+    def adjoint_workspace(self, nsteps, seg):
+        b = ctypes.c_longlong()
+        _check(self._lib.tg_adjoint_workspace(self._h, int(nsteps), int(seg), ctypes.byref(b)))
+        return torch.empty(int(b.value), dtype=torch.uint8, device=self.device)
+
+    def adjoint_forward(self, x0_hat, nsteps, dt, nu, objective, seg, aws):
+        self._spec(x0_hat, "x0_hat")
+        J = ctypes.c_double()
+        with torch.cuda.device(self.device):
+            _check(self._lib.tg_adjoint_forward(self._h, _ptr(x0_hat), int(nsteps), float(dt), float(nu),
+                                                int(objective), int(seg), ctypes.byref(J), _ptr(self.ws),
+                                                _ptr(aws), _stream()))
+        return float(J.value)
+
+    def adjoint_backward(self, nsteps, dt, nu, seg, aws):
+        grad = self.empty_spec()
+        d = ctypes.c_double()
+        with torch.cuda.device(self.device):
+            _check(self._lib.tg_adjoint_backward(self._h, int(nsteps), float(dt), float(nu), int(seg),
+                                                 _ptr(grad), ctypes.byref(d), _ptr(self.ws), _ptr(aws),
+                                                 _stream()))
+        return grad, float(d.value)
 
     def variant_count(self):
         cnt = ctypes.c_int()

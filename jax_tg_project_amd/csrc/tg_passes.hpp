@@ -64,12 +64,12 @@ struct CfgXF {
   static constexpr int NT = LINES * N / E;
 };
 
-enum XMode { XM_HYDRO = 0, XM_MHD = 1, XM_COPY3 = 2, XM_DIV = 3, XM_CURL3 = 4 };
-enum ZMode { ZM_HYDRO = 0, ZM_MHD = 1, ZM_PHYS_OUT3 = 2, ZM_MAXABS = 3, ZM_PHYS_IN3 = 4, ZM_PHYS_OUT1 = 5 };
-enum FMode { FM_HYDRO_STAGE = 0, FM_MHD_STAGE = 1, FM_STORE3 = 2, FM_HYDRO_RHS = 3, FM_MHD_RHS = 4 };
+enum XMode { XM_HYDRO = 0, XM_MHD = 1, XM_COPY3 = 2, XM_DIV = 3, XM_CURL3 = 4, XM_ADJ = 5 };
+enum ZMode { ZM_HYDRO = 0, ZM_MHD = 1, ZM_PHYS_OUT3 = 2, ZM_MAXABS = 3, ZM_PHYS_IN3 = 4, ZM_PHYS_OUT1 = 5, ZM_ADJ = 6 };
+enum FMode { FM_HYDRO_STAGE = 0, FM_MHD_STAGE = 1, FM_STORE3 = 2, FM_HYDRO_RHS = 3, FM_MHD_RHS = 4, FM_ADJ = 5 };
 
 __host__ __device__ constexpr int xmode_fields(int m) {
-  return m == XM_HYDRO ? 6 : m == XM_MHD ? 12 : (m == XM_COPY3 || m == XM_CURL3) ? 3 : 1;
+  return m == XM_HYDRO ? 6 : m == XM_MHD ? 12 : m == XM_ADJ ? 9 : (m == XM_COPY3 || m == XM_CURL3) ? 3 : 1;
 }
 
 // ----------------------------------------------------------------- pass 1
@@ -80,13 +80,26 @@ __host__ __device__ constexpr int xmode_fields(int m) {
 template <typename R, int N, int MODE>
 __device__ __forceinline__ cpx<R> xfield(int f, const cpx<R>* __restrict__ su,
                                          const cpx<R>* __restrict__ sb, size_t pt, size_t cs,
-                                         R kx, R ky, R kz) {
+                                         R kx, R ky, R kz, R alpha = R(1)) {
+  if constexpr (MODE == XM_ADJ) {
+    // fields 0-5: u, curl u of the state su; 6-8: P(alpha * a) of the
+    // cotangent sb (Leray projection, k2 == 0 -> 1)
+    if (f >= 6) {
+      const cpx<R> a = sb[pt], b = sb[cs + pt], c = sb[2 * cs + pt];
+      const R k2 = kx * kx + ky * ky + kz * kz;
+      const R k2s = (k2 == R(0)) ? R(1) : k2;
+      const cpx<R> kd = {kx * a.x + ky * b.x + kz * c.x, kx * a.y + ky * b.y + kz * c.y};
+      const R kc = f == 6 ? kx : (f == 7 ? ky : kz);
+      const cpx<R> v = f == 6 ? a : (f == 7 ? b : c);
+      return {alpha * (v.x - kc * kd.x / k2s), alpha * (v.y - kc * kd.y / k2s)};
+    }
+  }
   const cpx<R>* s = su;
   if constexpr (MODE == XM_MHD) {
     if (f >= 6) { s = sb; f -= 6; }
   }
   if constexpr (MODE == XM_CURL3) f += 3;
-  if (f < 3 && MODE != XM_DIV) return s[f * cs + pt];
+  if (f < 3 && MODE != XM_DIV) return s[f * cs + pt];  // (ADJ f < 6 falls through to the curl)
   const cpx<R> a = s[pt], b = s[cs + pt], c = s[2 * cs + pt];
   if constexpr (MODE == XM_DIV) {
     cpx<R> d = {kx * a.x + ky * b.x + kz * c.x, kx * a.y + ky * b.y + kz * c.y};
@@ -107,7 +120,7 @@ struct XInvBody {
   // Block (bx, by) = (kz tile, iy).
   __device__ static void run(cpx<R>* sm, int bx, int by, const cpx<R>* __restrict__ su,
                              const cpx<R>* __restrict__ sb, cpx<R>* __restrict__ out,
-                             const cpx<R>* __restrict__ tw, R ks) {
+                             const cpx<R>* __restrict__ tw, R ks, R alpha = R(1)) {
     constexpr int L = C::L, E = C::E;
     constexpr int F = xmode_fields(MODE);
     cpx<R>* buf = sm;
@@ -126,7 +139,8 @@ struct XInvBody {
         const int iz = ok ? iz0c : 0;
         const int ix = ok ? G::compact_of(n) : 0;
         const size_t pt = (size_t(ix) * G::K + iy) * G::KZ + iz;
-        const cpx<R> v = cscale(xfield<R, N, MODE>(f, su, sb, pt, cs, ks * R(G::kint(ix)), ky, ks * R(iz)), scale);
+        const cpx<R> v =
+            cscale(xfield<R, N, MODE>(f, su, sb, pt, cs, ks * R(G::kint(ix)), ky, ks * R(iz), alpha), scale);
         return ok ? v : cpx<R>{R(0), R(0)};
       };
       auto st = [&](int n, int ll, cpx<R> v) {
@@ -142,10 +156,11 @@ template <typename R, int N, int MODE, int V>
 __global__ __launch_bounds__((CfgXY<R, N, V>::NT)) void k_inv_x(const cpx<R>* __restrict__ su,
                                                               const cpx<R>* __restrict__ sb,
                                                               cpx<R>* __restrict__ out,
-                                                              const cpx<R>* __restrict__ tw, R ks) {
+                                                              const cpx<R>* __restrict__ tw, R ks,
+                                                              R alpha) {
   using B = XInvBody<R, N, MODE, V>;
   __shared__ cpx<R> sm[B::SMEM];
-  B::run(sm, blockIdx.x, blockIdx.y, su, sb, out, tw, ks);
+  B::run(sm, blockIdx.x, blockIdx.y, su, sb, out, tw, ks, alpha);
 }
 
 // ----------------------------------------------------------------- pass 2/4
@@ -213,12 +228,15 @@ template <> struct ZShape<ZM_PHYS_OUT3> { static constexpr int SPL = 2, FIN = 3,
 template <> struct ZShape<ZM_MAXABS> { static constexpr int SPL = 1, FIN = 1, FOUT = 0; };
 template <> struct ZShape<ZM_PHYS_IN3> { static constexpr int SPL = 3, FIN = 0, FOUT = 3; };
 template <> struct ZShape<ZM_PHYS_OUT1> { static constexpr int SPL = 1, FIN = 1, FOUT = 0; };
+// ADJ: 9 inverse fields (u, curl u, q) per line packed as 9 sequences per
+// pair of lines, 6 forward fields per line (SPL counts per 2 lines / 2).
+template <> struct ZShape<ZM_ADJ> { static constexpr int SPL = 0, FIN = 9, FOUT = 6; };
 
 // Per-mode block shape: LZ lines, NSEQ = LZ * SPL inverse-side sequences.
 template <typename R, int N, int MODE, int V>
 struct CfgZM {
-  static constexpr int LZ = (MODE == ZM_MHD) ? CfgZ<R, N, V>::NB / 6 : CfgZ<R, N, V>::NB / 3;
-  static constexpr int NSEQ = LZ * ZShape<MODE>::SPL;
+  static constexpr int LZ = (MODE == ZM_MHD || MODE == ZM_ADJ) ? CfgZ<R, N, V>::NB / 6 : CfgZ<R, N, V>::NB / 3;
+  static constexpr int NSEQ = (MODE == ZM_ADJ) ? 9 * LZ / 2 : LZ * ZShape<MODE>::SPL;
   static constexpr int E = CfgZ<R, N, V>::E;
   static constexpr int NT = NSEQ * N / E;
   static constexpr int LP = NSEQ | 1;  // odd pitch: conflict-free column access
@@ -237,7 +255,7 @@ struct ZBody {
   using Z = CfgZM<R, N, MODE, V>;
   using S = ZShape<MODE>;
   static constexpr int PSI = pad_shift<N, Z::E>();
-  static constexpr int PSF = (S::FOUT > 0) ? pad_shift<N, Z::E / 2>() : PSI;
+  static constexpr int PSF = (MODE == ZM_ADJ) ? PSI : (S::FOUT > 0) ? pad_shift<N, Z::E / 2>() : PSI;
   static constexpr int PSMIN = PSI < PSF ? PSI : PSF;
   static constexpr int BUF = N * Z::LP + (N >> PSMIN) + 1;
   static constexpr int SMEM = BUF + N;
@@ -262,6 +280,83 @@ struct ZBody {
   const size_t NNKZ = size_t(N) * N * KZ;
   const size_t line0 = size_t(x) * N + y0;  // (x, y0) line index
 
+  if constexpr (MODE == ZM_ADJ) {
+    // ---- inverse of u (f 0-2), curl u (3-5), q (6-8) for line pairs: per
+    // pair g, sequences j = 0..3 (line 2g), 4..7 (line 2g+1) pack (u0,u1),
+    // (u2,w0), (w1,w2), (q0,q1); j = 8 packs (q2 of 2g, q2 of 2g+1).
+    auto ld = [&](int k, int s) -> cpx<R> {
+      const int g = s / 9, j = s % 9;
+      int la, lb, fa, fb;
+      if (j < 8) {
+        la = lb = 2 * g + (j >> 2);
+        fa = 2 * (j & 3);
+        fb = fa + 1;
+      } else {
+        la = 2 * g; lb = 2 * g + 1; fa = fb = 8;
+      }
+      const bool neg = k > N / 2;
+      const int kzr = neg ? N - k : k;
+      const bool ok = kzr < KZ;
+      const int kz = ok ? kzr : 0;
+      cpx<R> a = in[fa * NNKZ + (line0 + la) * KZ + kz];
+      cpx<R> b = in[fb * NNKZ + (line0 + lb) * KZ + kz];
+      if (k == 0) { a.y = R(0); b.y = R(0); }
+      if (neg) { a.y = -a.y; b.y = -b.y; }
+      const cpx<R> z = {a.x - b.y, a.y + b.x};
+      return ok ? z : cpx<R>{R(0), R(0)};
+    };
+    fft_io<R, N, E, +1, 1, false, true, PSI>(buf, LP, NSEQ, tws, ld, NoIO{});
+    // ---- products per line: p1 = w x q, p2 = q x u -> 3 forward sequences
+    // per line: (p1x,p1y), (p1z,p2x), (p2y,p2z)
+    constexpr int ITEMS = (LZ / 2) * N;
+    constexpr int IPT = (ITEMS + NT - 1) / NT;
+    R res[IPT][12];
+#pragma unroll
+    for (int it = 0; it < IPT; ++it) {
+      const int item = tid + it * NT;
+      if (!live || item >= ITEMS) break;
+      const int z = item % N, g = item / N;
+      const cpx<R> q2 = bi(z, 9 * g + 8);
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const cpx<R> s0 = bi(z, 9 * g + 4 * h), s1 = bi(z, 9 * g + 4 * h + 1);
+        const cpx<R> s2 = bi(z, 9 * g + 4 * h + 2), s3 = bi(z, 9 * g + 4 * h + 3);
+        const R u[3] = {s0.x, s0.y, s1.x};
+        const R w[3] = {s1.y, s2.x, s2.y};
+        const R q[3] = {s3.x, s3.y, h ? q2.y : q2.x};
+        cross3(w, q, &res[it][6 * h]);
+        cross3(q, u, &res[it][6 * h + 3]);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < IPT; ++it) {
+      const int item = tid + it * NT;
+      if (!live || item >= ITEMS) break;
+      const int z = item % N, g = item / N;
+      const R* r = res[it];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int line = 2 * g + h;
+        bf(z, 3 * line + 0) = {r[6 * h + 0], r[6 * h + 1]};
+        bf(z, 3 * line + 1) = {r[6 * h + 2], r[6 * h + 3]};
+        bf(z, 3 * line + 2) = {r[6 * h + 4], r[6 * h + 5]};
+      }
+    }
+    __syncthreads();
+    fft_io<R, N, E, -1, 0, false, true, PSF>(buf, LP, 3 * LZ, tws, NoIO{}, NoIO{});
+    for (int idx = tid; live && idx < 3 * LZ * KZ; idx += NT) {
+      const int kz = idx % KZ, q = idx / KZ;
+      const cpx<R> P = bf(kz, q);
+      const cpx<R> Qc = bf((N - kz) % N, q);
+      const cpx<R> re = {R(0.5) * (P.x + Qc.x), R(0.5) * (P.y - Qc.y)};
+      const cpx<R> im = {R(0.5) * (P.y + Qc.y), R(0.5) * (Qc.x - P.x)};
+      const int line = q / 3, p = q % 3;
+      out[(2 * p) * NNKZ + (line0 + line) * KZ + kz] = re;
+      out[(2 * p + 1) * NNKZ + (line0 + line) * KZ + kz] = im;
+    }
+    return;
+  }
   if constexpr (MODE != ZM_PHYS_IN3) {
     // ---- inverse: sequence s = line*SPL + p packs fields (fa, fb) as a + i b;
     // the first FFT stage reads the Hermitian-extended coefficients directly.
@@ -438,6 +533,18 @@ struct StageArgs {
   cpx<R>* bacc;
   R dt, nu, eta, ks;
   int stage;
+  // adjoint (FM_ADJ): mu = P-less VJP of the RHS at the stage state xs for
+  // the cotangent a_s = alpha * a;  lacc <- (lamp | lacc | 0) + mu by
+  // `first` (1 | 0 | 2);  anext <- clam * lamp + cmu * mu unless `last`;
+  // nupart[block] += sum w k^2 Re(conj(a_s) xs)
+  const cpx<R>* a;
+  const cpx<R>* lamp;
+  cpx<R>* lacc;
+  const cpx<R>* xs;
+  cpx<R>* anext;
+  R alpha, clam, cmu;
+  int first, last;
+  double* nupart;
 };
 
 template <typename R>
@@ -481,8 +588,9 @@ struct XFBody {
   // Block (bx, by) = (kz tile, iy).
   __device__ static void run(cpx<R>* sm, int bx, int by, const cpx<R>* __restrict__ in, const StageArgs<R>& a,
                              const cpx<R>* __restrict__ tw) {
-  constexpr int FF = (MODE == FM_MHD_STAGE || MODE == FM_MHD_RHS) ? 6 : 3;
+  constexpr int FF = (MODE == FM_MHD_STAGE || MODE == FM_MHD_RHS || MODE == FM_ADJ) ? 6 : 3;
   constexpr int NT = CfgXF<R, N, V>::NT, E = CfgXF<R, N, V>::E;
+  double nusum = 0.0;
   constexpr int L = LINES / FF;
   constexpr int LP = LINES;
   cpx<R>* buf = sm;
@@ -510,6 +618,33 @@ struct XFBody {
     if constexpr (MODE == FM_STORE3) {
 #pragma unroll
       for (int c = 0; c < 3; ++c) a.unext[c * cs + pt] = v[c];
+      continue;
+    } else if constexpr (MODE == FM_ADJ) {
+      const R kx = a.ks * R(G::kint(ix)), kz = a.ks * R(iz);
+      const R k2 = kx * kx + ky * ky + kz * kz;
+      const R nk = a.nu * k2;
+      cpx<R> cb[3];
+      cb[0] = cmuli(cpx<R>{ky * v[5].x - kz * v[4].x, ky * v[5].y - kz * v[4].y});
+      cb[1] = cmuli(cpx<R>{kz * v[3].x - kx * v[5].x, kz * v[3].y - kx * v[5].y});
+      cb[2] = cmuli(cpx<R>{kx * v[4].x - ky * v[3].x, kx * v[4].y - ky * v[3].y});
+      const double wk = (iz == 0 ? 1.0 : 2.0) * double(k2);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const size_t o = c * cs + pt;
+        const cpx<R> ar = a.a[o];
+        const cpx<R> as = {a.alpha * ar.x, a.alpha * ar.y};
+        const cpx<R> mu = {v[c].x + cb[c].x - nk * as.x, v[c].y + cb[c].y - nk * as.y};
+        const cpx<R> lp = (a.first == 1 || !a.last) ? a.lamp[o] : cpx<R>{R(0), R(0)};
+        cpx<R> acc = mu;
+        if (a.first == 1) acc = {lp.x + mu.x, lp.y + mu.y};
+        else if (a.first == 0) { const cpx<R> l0 = a.lacc[o]; acc = {l0.x + mu.x, l0.y + mu.y}; }
+        if (a.xs) {
+          const cpx<R> xv = a.xs[o];
+          nusum += wk * (double(as.x) * double(xv.x) + double(as.y) * double(xv.y));
+        }
+        a.lacc[o] = acc;
+        if (!a.last) a.anext[o] = {a.clam * lp.x + a.cmu * mu.x, a.clam * lp.y + a.cmu * mu.y};
+      }
       continue;
     } else {
       const R kx = a.ks * R(G::kint(ix)), kz = a.ks * R(iz);
@@ -556,6 +691,19 @@ struct XFBody {
       }
     }
   }
+  if constexpr (MODE == FM_ADJ) {
+    if (a.nupart) {
+      __syncthreads();  // every thread is done reading buf
+      double* red = reinterpret_cast<double*>(buf);
+      red[threadIdx.x] = nusum;
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int i = 0; i < int(blockDim.x); ++i) t += red[i];
+        a.nupart[size_t(by) * gridDim.x + bx] += t;
+      }
+    }
+  }
   }
 };
 
@@ -566,6 +714,45 @@ __global__ __launch_bounds__((CfgXF<R, N, V>::NT)) void k_fwd_x(const cpx<R>* __
   using B = XFBody<R, N, MODE, V>;
   __shared__ cpx<R> sm[B::SMEM];
   B::run(sm, blockIdx.x, blockIdx.y, in, a, tw);
+}
+
+// ----------------------------------------------------------------- adjoint helpers
+// Cotangent of the final state for J = E (objective 0) or Z (1), as the
+// unnormalised spectrum of the physical gradient: dE/du = u / N^3,
+// dZ/du = curl curl u / N^3 = (k^2 u - k (k.u)) / N^3.
+template <typename R, int N>
+__global__ __launch_bounds__(256) void k_obj_grad(const cpx<R>* __restrict__ x, cpx<R>* __restrict__ lam,
+                                                  int objective, R ks) {
+  using G = Geo<N>;
+  const size_t cs = size_t(G::K) * G::K * G::KZ;
+  const size_t pt = size_t(blockIdx.x) * 256 + threadIdx.x;
+  if (pt >= cs) return;
+  const int iz = int(pt % G::KZ), iy = int((pt / G::KZ) % G::K), ix = int(pt / (size_t(G::KZ) * G::K));
+  const R sc = R(1) / (R(N) * R(N) * R(N));
+  cpx<R> v[3] = {x[pt], x[cs + pt], x[2 * cs + pt]};
+  if (objective == 1) {
+    const R kx = ks * R(G::kint(ix)), ky = ks * R(G::kint(iy)), kz = ks * R(iz);
+    const R k2 = kx * kx + ky * ky + kz * kz;
+    const cpx<R> kd = {kx * v[0].x + ky * v[1].x + kz * v[2].x, kx * v[0].y + ky * v[1].y + kz * v[2].y};
+    const R kk[3] = {kx, ky, kz};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) v[c] = {k2 * v[c].x - kk[c] * kd.x, k2 * v[c].y - kk[c] * kd.y};
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) lam[c * cs + pt] = cscale(v[c], sc);
+}
+
+// Physical-cotangent spectrum -> gradient w.r.t. the stored compact entries
+// (d/dRe + i d/dIm): weight (1 at kz = 0, 2 otherwise) / N^3.
+template <typename R, int N>
+__global__ __launch_bounds__(256) void k_adj_final(const cpx<R>* __restrict__ lam, cpx<R>* __restrict__ g) {
+  using G = Geo<N>;
+  const size_t cs = size_t(G::K) * G::K * G::KZ;
+  const size_t pt = size_t(blockIdx.x) * 256 + threadIdx.x;
+  if (pt >= cs) return;
+  const R w = (pt % G::KZ == 0 ? R(1) : R(2)) / (R(N) * R(N) * R(N));
+#pragma unroll
+  for (int c = 0; c < 3; ++c) g[c * cs + pt] = cscale(lam[c * cs + pt], w);
 }
 
 // ----------------------------------------------------------------- diagnostics

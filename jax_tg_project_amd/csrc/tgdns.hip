@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -115,17 +116,18 @@ template <typename R, int N>
 struct Launch {
   using G = Geo<N>;
   static int inv_x(int mode, int var, const cpx<R>* su, const cpx<R>* sb, cpx<R>* out, const cpx<R>* tw, R ks,
-                   hipStream_t s) {
+                   hipStream_t s, R al = R(1)) {
     return with_variant<N>(var, [&](auto vc) {
       constexpr int V = decltype(vc)::value;
       using C = CfgXY<R, N, V>;
       dim3 grid(cdiv(G::KZ, C::L), G::K), blk(C::NT);
       switch (mode) {
-        case XM_HYDRO: hipLaunchKernelGGL((k_inv_x<R, N, XM_HYDRO, V>), grid, blk, 0, s, su, sb, out, tw, ks); break;
-        case XM_MHD: hipLaunchKernelGGL((k_inv_x<R, N, XM_MHD, V>), grid, blk, 0, s, su, sb, out, tw, ks); break;
-        case XM_COPY3: hipLaunchKernelGGL((k_inv_x<R, N, XM_COPY3, V>), grid, blk, 0, s, su, sb, out, tw, ks); break;
-        case XM_CURL3: hipLaunchKernelGGL((k_inv_x<R, N, XM_CURL3, V>), grid, blk, 0, s, su, sb, out, tw, ks); break;
-        default: hipLaunchKernelGGL((k_inv_x<R, N, XM_DIV, V>), grid, blk, 0, s, su, sb, out, tw, ks); break;
+        case XM_HYDRO: hipLaunchKernelGGL((k_inv_x<R, N, XM_HYDRO, V>), grid, blk, 0, s, su, sb, out, tw, ks, al); break;
+        case XM_MHD: hipLaunchKernelGGL((k_inv_x<R, N, XM_MHD, V>), grid, blk, 0, s, su, sb, out, tw, ks, al); break;
+        case XM_COPY3: hipLaunchKernelGGL((k_inv_x<R, N, XM_COPY3, V>), grid, blk, 0, s, su, sb, out, tw, ks, al); break;
+        case XM_CURL3: hipLaunchKernelGGL((k_inv_x<R, N, XM_CURL3, V>), grid, blk, 0, s, su, sb, out, tw, ks, al); break;
+        case XM_ADJ: hipLaunchKernelGGL((k_inv_x<R, N, XM_ADJ, V>), grid, blk, 0, s, su, sb, out, tw, ks, al); break;
+        default: hipLaunchKernelGGL((k_inv_x<R, N, XM_DIV, V>), grid, blk, 0, s, su, sb, out, tw, ks, al); break;
       }
       TG_LAUNCHED();
       return TG_OK;
@@ -166,6 +168,7 @@ struct Launch {
       case ZM_PHYS_OUT3: return z_<ZM_PHYS_OUT3, 0>(in, out, pin, pout, tw, lor, s);
       case ZM_MAXABS: return z_<ZM_MAXABS, 0>(in, out, pin, pout, tw, lor, s);
       case ZM_PHYS_OUT1: return z_<ZM_PHYS_OUT1, 0>(in, out, pin, pout, tw, lor, s);
+      case ZM_ADJ: return z_<ZM_ADJ, 0>(in, out, pin, pout, tw, lor, s);
       default: return z_<ZM_PHYS_IN3, 0>(in, out, pin, pout, tw, lor, s);
     }
   }
@@ -174,7 +177,7 @@ struct Launch {
     return with_variant<N>(var, [&](auto vc) {
       constexpr int V = decltype(vc)::value;
       using C = CfgXF<R, N, V>;
-      const int ff = (mode == FM_MHD_STAGE || mode == FM_MHD_RHS) ? 6 : 3;
+      const int ff = (mode == FM_MHD_STAGE || mode == FM_MHD_RHS || mode == FM_ADJ) ? 6 : 3;
       const int L = C::LINES / ff;
       dim3 grid(cdiv(G::KZ, L), G::K), blk(C::NT);
       switch (mode) {
@@ -182,10 +185,18 @@ struct Launch {
         case FM_MHD_STAGE: hipLaunchKernelGGL((k_fwd_x<R, N, FM_MHD_STAGE, V>), grid, blk, 0, s, in, a, tw); break;
         case FM_STORE3: hipLaunchKernelGGL((k_fwd_x<R, N, FM_STORE3, V>), grid, blk, 0, s, in, a, tw); break;
         case FM_HYDRO_RHS: hipLaunchKernelGGL((k_fwd_x<R, N, FM_HYDRO_RHS, V>), grid, blk, 0, s, in, a, tw); break;
+        case FM_ADJ: hipLaunchKernelGGL((k_fwd_x<R, N, FM_ADJ, V>), grid, blk, 0, s, in, a, tw); break;
         default: hipLaunchKernelGGL((k_fwd_x<R, N, FM_MHD_RHS, V>), grid, blk, 0, s, in, a, tw); break;
       }
       TG_LAUNCHED();
       return TG_OK;
+    });
+  }
+  // blocks of the forward-x pass (slots of the adjoint nu-gradient partials)
+  static int fwd_x_blocks(int var, bool six) {
+    return with_variant<N>(var, [&](auto vc) {
+      using C = CfgXF<R, N, decltype(vc)::value>;
+      return cdiv(G::KZ, C::LINES / (six ? 6 : 3)) * G::K;
     });
   }
   static int diag(const cpx<R>* su, const cpx<R>* sb, double* part, int nblk, double ks, hipStream_t s) {
@@ -305,6 +316,130 @@ struct Ops {
         TG_RET(rhs_passes<R, N>(p, w, a.us, a.bs, fm, a, R(lor), s));
       }
     }
+    return TG_OK;
+  }
+  // One vector-Jacobian product of the hydro RHS at x for the cotangent a
+  // (scaled by a.alpha), with the epilogue bookkeeping described in `a`.
+  static int vjp_passes(const tg_plan* p, const Ptrs<R>& w, const cpx<R>* x, const StageArgs<R>& a,
+                        hipStream_t s) {
+    using Lc = Launch<R, N>;
+    const cpx<R>* tw = static_cast<const cpx<R>*>(p->tw);
+    TG_TIMED(TG_PASS_INV_X, Lc::inv_x(XM_ADJ, 0, x, a.a, w.A, tw, R(p->ks), s, a.alpha));
+    TG_TIMED(TG_PASS_INV_Y, Lc::y(+1, 0, 9, w.A, w.B, tw, s));
+    TG_TIMED(TG_PASS_Z, Lc::z(ZM_ADJ, 0, w.B, w.A, nullptr, nullptr, tw, R(0), s));
+    TG_TIMED(TG_PASS_FWD_Y, Lc::y(-1, 0, 6, w.A, w.B, tw, s));
+    TG_TIMED(TG_PASS_FWD_X, Lc::fwd_x(FM_ADJ, 0, w.B, a, tw, s));
+    return TG_OK;
+  }
+  static int vjp(const tg_plan* p, const cpx<R>* x, const cpx<R>* av, double nu, cpx<R>* out, void* ws,
+                 hipStream_t s) {
+    Ptrs<R> w = carve<R>(p, ws);
+    StageArgs<R> a{};
+    a.nu = R(nu); a.ks = R(p->ks);
+    a.a = av; a.alpha = R(1); a.lacc = out; a.first = 2; a.last = 1;
+    return vjp_passes(p, w, x, a, s);
+  }
+  // Adjoint workspace: cur | lamp | lacc | abuf | nck checkpoints | 4 seg
+  // stage states | nu-gradient partials.
+  struct AdjLayout {
+    cpx<R>* cur; cpx<R>* lamp; cpx<R>* lacc; cpx<R>* abuf; cpx<R>* ck; cpx<R>* st; double* nupart;
+    size_t vstride; int nck, nblk;
+  };
+  static AdjLayout adj_layout(const tg_plan* p, int nsteps, int seg, void* aws) {
+    AdjLayout L;
+    const size_t vb = align256(3 * p->compact * sizeof(cpx<R>));
+    L.nck = (nsteps + seg - 1) / seg;
+    char* base = static_cast<char*>(aws);
+    L.cur = reinterpret_cast<cpx<R>*>(base);
+    L.lamp = reinterpret_cast<cpx<R>*>(base + vb);
+    L.lacc = reinterpret_cast<cpx<R>*>(base + 2 * vb);
+    L.abuf = reinterpret_cast<cpx<R>*>(base + 3 * vb);
+    L.ck = reinterpret_cast<cpx<R>*>(base + 4 * vb);
+    L.st = reinterpret_cast<cpx<R>*>(base + (4 + L.nck) * vb);
+    L.nupart = reinterpret_cast<double*>(base + (4 + L.nck + 4 * size_t(seg)) * vb);
+    L.vstride = vb / sizeof(cpx<R>);
+    L.nblk = Launch<R, N>::fwd_x_blocks(0, true);
+    return L;
+  }
+  // Forward sweep: nsteps RK4 steps from x0 with a checkpoint every `seg`
+  // steps, J = E(T) (objective 0) or Z(T) (1), and the terminal cotangent.
+  static int adj_forward(const tg_plan* p, const cpx<R>* x0, int nsteps, double dt, double nu, int objective,
+                         int seg, double* J, void* ws, void* aws, hipStream_t s) {
+    AdjLayout L = adj_layout(p, nsteps, seg, aws);
+    const size_t bytes = 3 * p->compact * sizeof(cpx<R>);
+    TG_HIP(hipMemcpyAsync(L.cur, x0, bytes, hipMemcpyDeviceToDevice, s));
+    for (int c = 0; c < L.nck; ++c) {
+      TG_HIP(hipMemcpyAsync(L.ck + c * L.vstride, L.cur, bytes, hipMemcpyDeviceToDevice, s));
+      TG_RET(rk4(p, L.cur, nullptr, std::min(seg, nsteps - c * seg), dt, nu, 0.0, -1.0, ws, s));
+    }
+    double d[NDIAG + 2];
+    TG_RET(diagnostics(p, L.cur, nullptr, 0, d, ws, s));
+    *J = objective == 0 ? d[0] : d[1];
+    hipLaunchKernelGGL((k_obj_grad<R, N>), dim3(cdiv(int(p->compact), 256)), dim3(256), 0, s, L.cur, L.lamp,
+                       objective, R(p->ks));
+    TG_LAUNCHED();
+    return TG_OK;
+  }
+  // Reverse sweep: per segment (last first) recompute its steps from the
+  // checkpoint keeping the four stage states of each step, then four
+  // vector-Jacobian products per step in reverse.  grad = dJ/d(stored
+  // entries of x0), dJ/dnu to the host.
+  static int adj_backward(const tg_plan* p, int nsteps, double dt, double nu, int seg, cpx<R>* grad,
+                          double* dJdnu, void* ws, void* aws, hipStream_t s) {
+    Ptrs<R> w = carve<R>(p, ws);
+    AdjLayout L = adj_layout(p, nsteps, seg, aws);
+    const size_t bytes = 3 * p->compact * sizeof(cpx<R>);
+    TG_HIP(hipMemsetAsync(L.nupart, 0, L.nblk * sizeof(double), s));
+    cpx<R>* lamp = L.lamp;
+    cpx<R>* lacc = L.lacc;
+    const R h = R(dt);
+    const R cl[4] = {h / R(6), h / R(3), h / R(3), h / R(6)};  // weight of lamp in a_s (s = 1..4)
+    const R cm[4] = {R(0), h / R(2), h / R(2), h};              // weight of mu_{s+1} in a_s
+    for (int c = L.nck - 1; c >= 0; --c) {
+      const int len = std::min(seg, nsteps - c * seg);
+      TG_HIP(hipMemcpyAsync(L.cur, L.ck + c * L.vstride, bytes, hipMemcpyDeviceToDevice, s));
+      for (int j = 0; j < len; ++j) {
+        cpx<R>* S = L.st + size_t(4 * j) * L.vstride;
+        TG_HIP(hipMemcpyAsync(S, L.cur, bytes, hipMemcpyDeviceToDevice, s));
+        for (int stg = 0; stg < 4; ++stg) {
+          StageArgs<R> a{};
+          a.u0 = L.cur;
+          a.us = S + size_t(stg) * L.vstride;
+          a.unext = stg == 3 ? L.cur : S + size_t(stg + 1) * L.vstride;
+          a.uacc = w.Acc;
+          a.dt = R(dt); a.nu = R(nu); a.ks = R(p->ks); a.stage = stg;
+          TG_RET(rhs_passes<R, N>(p, w, a.us, nullptr, FM_HYDRO_STAGE, a, R(0), s));
+        }
+      }
+      for (int j = len - 1; j >= 0; --j) {
+        cpx<R>* S = L.st + size_t(4 * j) * L.vstride;
+        for (int sidx = 3; sidx >= 0; --sidx) {
+          StageArgs<R> a{};
+          a.nu = R(nu); a.ks = R(p->ks);
+          a.lamp = lamp; a.lacc = lacc; a.xs = S + size_t(sidx) * L.vstride;
+          a.nupart = L.nupart;
+          if (sidx == 3) { a.a = lamp; a.alpha = cl[3]; a.first = 1; }
+          else { a.a = L.abuf; a.alpha = R(1); a.first = 0; }
+          a.last = sidx == 0;
+          if (sidx > 0) { a.anext = L.abuf; a.clam = cl[sidx - 1]; a.cmu = cm[sidx]; }
+          TG_RET(vjp_passes(p, w, a.xs, a, s));
+        }
+        std::swap(lamp, lacc);
+      }
+    }
+    hipLaunchKernelGGL((k_adj_final<R, N>), dim3(cdiv(int(p->compact), 256)), dim3(256), 0, s, lamp, grad);
+    TG_LAUNCHED();
+    std::vector<double> part(L.nblk);
+    TG_HIP(hipMemcpyAsync(part.data(), L.nupart, L.nblk * sizeof(double), hipMemcpyDeviceToHost, s));
+    TG_HIP(hipStreamSynchronize(s));
+    double sum = 0.0, comp = 0.0;
+    for (double v : part) {
+      const double t = sum + v;
+      comp += std::fabs(sum) >= std::fabs(v) ? (sum - t) + v : (v - t) + sum;
+      sum = t;
+    }
+    const double n3 = double(p->n) * p->n * p->n;
+    *dJdnu = -(sum + comp) / n3;
     return TG_OK;
   }
   static int diagnostics(const tg_plan* p, const cpx<R>* u, const cpx<R>* b, int divmax, double* out,
@@ -429,8 +564,10 @@ int tg_plan_create(int n, int precision_bits, int mhd, double domain_length, tg_
   p->cb = precision_bits == 64 ? 16 : 8;
   p->compact = size_t(p->K) * p->K * p->KZ;
   const size_t N = n;
-  const size_t fi = p->mhd ? 12 : 6;
-  const size_t sA = fi * N * p->K * p->KZ * p->cb;
+  const size_t fi = p->mhd ? 12 : 9;  // inverse fields: hydro RHS 6, adjoint 9, MHD 12
+  const size_t fo = 6;                 // z-pass outputs: hydro 3, adjoint and MHD 6
+  // A holds the inverse-x output (fi x N K KZ) and later the z output (fo x N N KZ)
+  const size_t sA = std::max(fi * N * p->K, fo * N * N) * p->KZ * p->cb;
   const size_t sB = fi * N * N * p->KZ * p->cb;
   const size_t sU = (p->mhd ? 2 : 1) * 3 * p->compact * p->cb;
   p->nDiagBlk = 512;
@@ -550,6 +687,53 @@ int tg_diagnostics(tg_plan* p, const void* u_hat, const void* b_hat, int with_di
     using R = std::remove_pointer_t<decltype(rt)>;
     return decltype(ops)::diagnostics(p, static_cast<const cpx<R>*>(u_hat), static_cast<const cpx<R>*>(b_hat),
                                       with_divmax, out, ws, static_cast<hipStream_t>(stream));
+  });
+}
+
+int tg_vjp(tg_plan* p, const void* x_hat, const void* a_hat, double nu, void* out_hat, void* ws, void* stream) {
+  TG_RET(check_plan(p));
+  if (p->mhd) return fail(TG_ERR_UNSUPPORTED, "tg_vjp: hydro plans only");
+  if (!x_hat || !a_hat || !out_hat || !ws) return fail(TG_ERR_ARG, "tg_vjp: NULL buffer");
+  return with_plan(p, [&](auto ops, auto* rt) {
+    using R = std::remove_pointer_t<decltype(rt)>;
+    return decltype(ops)::vjp(p, static_cast<const cpx<R>*>(x_hat), static_cast<const cpx<R>*>(a_hat), nu,
+                              static_cast<cpx<R>*>(out_hat), ws, static_cast<hipStream_t>(stream));
+  });
+}
+
+int tg_adjoint_workspace(tg_plan* p, int nsteps, int seg, long long* bytes) {
+  TG_RET(check_plan(p));
+  if (nsteps < 1 || seg < 1 || !bytes) return fail(TG_ERR_ARG, "tg_adjoint_workspace: need nsteps, seg >= 1");
+  const size_t vb = align256(3 * p->compact * p->cb);
+  const size_t nck = (nsteps + seg - 1) / seg;
+  *bytes = (long long)((4 + nck + 4 * size_t(seg)) * vb + align256(size_t(p->K) * p->KZ * 8 * sizeof(double)));
+  return TG_OK;
+}
+
+int tg_adjoint_forward(tg_plan* p, const void* x0_hat, int nsteps, double dt, double nu, int objective, int seg,
+                       double* J, void* ws, void* adj_ws, void* stream) {
+  TG_RET(check_plan(p));
+  if (p->mhd) return fail(TG_ERR_UNSUPPORTED, "adjoint: hydro plans only");
+  if (!x0_hat || !J || !ws || !adj_ws) return fail(TG_ERR_ARG, "tg_adjoint_forward: NULL buffer");
+  if (nsteps < 1 || seg < 1) return fail(TG_ERR_ARG, "adjoint: need nsteps, seg >= 1");
+  if (objective != 0 && objective != 1) return fail(TG_ERR_ARG, "objective must be 0 (energy) or 1 (enstrophy)");
+  return with_plan(p, [&](auto ops, auto* rt) {
+    using R = std::remove_pointer_t<decltype(rt)>;
+    return decltype(ops)::adj_forward(p, static_cast<const cpx<R>*>(x0_hat), nsteps, dt, nu, objective, seg, J,
+                                      ws, adj_ws, static_cast<hipStream_t>(stream));
+  });
+}
+
+int tg_adjoint_backward(tg_plan* p, int nsteps, double dt, double nu, int seg, void* grad_hat, double* dJdnu,
+                        void* ws, void* adj_ws, void* stream) {
+  TG_RET(check_plan(p));
+  if (p->mhd) return fail(TG_ERR_UNSUPPORTED, "adjoint: hydro plans only");
+  if (!grad_hat || !dJdnu || !ws || !adj_ws) return fail(TG_ERR_ARG, "tg_adjoint_backward: NULL buffer");
+  if (nsteps < 1 || seg < 1) return fail(TG_ERR_ARG, "adjoint: need nsteps, seg >= 1");
+  return with_plan(p, [&](auto ops, auto* rt) {
+    using R = std::remove_pointer_t<decltype(rt)>;
+    return decltype(ops)::adj_backward(p, nsteps, dt, nu, seg, static_cast<cpx<R>*>(grad_hat), dJdnu, ws, adj_ws,
+                                       static_cast<hipStream_t>(stream));
   });
 }
 
