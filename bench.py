@@ -138,7 +138,6 @@ def run_adjoint(N, steps, dt, nu, dtype):
     plan = Plan(N, dtype=dtype)
     x0 = plan.forward(torch.tensor(ensemble.perturbed_taylor_green(N, 2024), dtype=dtype, device=plan.device))
     seg = sqrt_segment(steps)
-    aws = plan.adjoint_workspace(steps, seg)
     y = x0.clone()
     plan.rk4(y, nsteps=2, dt=dt, nu=nu)  # warm-up
     J = plan.adjoint_forward(x0, 2, dt, nu, 1, 1, plan.adjoint_workspace(2, 1))
@@ -148,14 +147,19 @@ def run_adjoint(N, steps, dt, nu, dtype):
     plan.rk4(y, nsteps=steps, dt=dt, nu=nu)
     torch.cuda.synchronize()
     fwd = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    J = plan.adjoint_forward(x0, steps, dt, nu, 1, seg, aws)
-    g, dnu = plan.adjoint_backward(steps, dt, nu, seg, aws)
-    torch.cuda.synchronize()
-    adj = time.perf_counter() - t0
-    return {"N": N, "steps": steps, "segment": seg, "dtype": "f64" if dtype == torch.float64 else "f32",
-            "forward_s": fwd, "forward_plus_adjoint_s": adj, "adjoint_over_forward": adj / fwd,
-            "J_enstrophy": J, "dJ_dnu": dnu}
+    out = {"N": N, "steps": steps, "dtype": "f64" if dtype == torch.float64 else "f32", "forward_s": fwd}
+    for name, sg in (("sqrt_n", seg), ("all_stages_kept", steps)):
+        aws = plan.adjoint_workspace(steps, sg)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        J = plan.adjoint_forward(x0, steps, dt, nu, 1, sg, aws)
+        g, dnu = plan.adjoint_backward(steps, dt, nu, sg, aws)
+        torch.cuda.synchronize()
+        adj = time.perf_counter() - t0
+        out[name] = {"segment": sg, "forward_plus_adjoint_s": adj, "adjoint_over_forward": adj / fwd,
+                     "workspace_GB": aws.numel() / 1e9, "J_enstrophy": J, "dJ_dnu": dnu}
+        del aws
+    return out
 
 
 def main():

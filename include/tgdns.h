@@ -172,13 +172,15 @@ int tg_vjp(tg_plan* plan, const void* x_hat, const void* a_hat, double nu, void*
 int tg_adjoint_workspace(tg_plan* plan, int nsteps, int seg, long long* bytes);
 /* Forward sweep: J = E(T) (objective 0) or Z(T) (1) after nsteps RK4 steps of
  * size dt from x0_hat with viscosity nu, keeping a checkpoint every `seg`
- * steps (seg = ceil(sqrt(nsteps)) is the sqrt-n schedule) in adj_ws.
+ * steps (seg = ceil(sqrt(nsteps)) is the sqrt-n schedule) in adj_ws, and
+ * the stage states of the last segment (so the reverse sweep never
+ * recomputes it; seg = nsteps keeps every stage and recomputes nothing).
  * Synchronises `stream` (J is returned to the host). */
 int tg_adjoint_forward(tg_plan* plan, const void* x0_hat, int nsteps, double dt, double nu, int objective, int seg,
                        double* J, void* ws, void* adj_ws, void* stream);
 /* Reverse sweep after tg_adjoint_forward with the same nsteps/dt/nu/seg and
- * adj_ws: each segment's steps are recomputed from its checkpoint once,
- * keeping their stage states, then reversed with four vector-Jacobian
+ * adj_ws: each earlier segment's steps are recomputed from its checkpoint
+ * once, keeping their stage states, then reversed with four vector-Jacobian
  * products per step.  grad_hat receives dJ/d(stored entries of x0_hat) as
  * d/dRe + i d/dIm (the convention of torch's complex .grad); *dJdnu the
  * viscosity derivative.  Synchronises `stream`. */

@@ -26,6 +26,18 @@ def sqrt_segment(nsteps):
     return max(1, int(math.ceil(math.sqrt(nsteps))))
 
 
+def choose_segment(plan, nsteps, budget_bytes=None):
+    """Checkpoint segment length: every stage state of the whole trajectory
+    (seg = nsteps: nothing recomputed) when 4 * nsteps state vectors fit in
+    `budget_bytes` (default: a quarter of the free device memory), else the
+    sqrt(n) schedule."""
+    vec = 3 * plan.K * plan.K * plan.KZ * (16 if plan.dtype == torch.float64 else 8)
+    if budget_bytes is None:
+        free, _ = torch.cuda.mem_get_info(plan.device)
+        budget_bytes = free // 4
+    return nsteps if 4 * nsteps * vec <= budget_bytes else sqrt_segment(nsteps)
+
+
 class FinalTimeObjective(torch.autograd.Function):
     @staticmethod
     def forward(ctx, u_hat0, nu, plan, nsteps, dt, objective, seg):
@@ -50,7 +62,9 @@ class FinalTimeObjective(torch.autograd.Function):
 
 def final_objective(u_hat0, nu, nsteps, dt, objective="enstrophy", seg=None, L=2.0 * math.pi):
     """J = E(T) or Z(T) after `nsteps` RK4 steps from the compact spectrum
-    u_hat0 (complex128 -> FP64 plan, complex64 -> FP32 plan)."""
+    u_hat0 (complex128 -> FP64 plan, complex64 -> FP32 plan).  `seg`:
+    checkpoint interval; None picks choose_segment(); "sqrt" forces the
+    sqrt(n) schedule."""
     dtype = torch.float64 if u_hat0.dtype == torch.complex128 else torch.float32
     M = (u_hat0.shape[-1] - 1)
     N = None
@@ -61,5 +75,8 @@ def final_objective(u_hat0, nu, nsteps, dt, objective="enstrophy", seg=None, L=2
     if N is None:
         raise ValueError(f"no supported grid for a compact spectrum of shape {tuple(u_hat0.shape)}")
     plan = get_plan(N, dtype, False, L, u_hat0.device)
-    return FinalTimeObjective.apply(u_hat0, nu, plan, int(nsteps), float(dt), OBJECTIVES[objective],
-                                    int(seg or sqrt_segment(nsteps)))
+    if seg is None:
+        seg = choose_segment(plan, nsteps)
+    elif seg == "sqrt":
+        seg = sqrt_segment(nsteps)
+    return FinalTimeObjective.apply(u_hat0, nu, plan, int(nsteps), float(dt), OBJECTIVES[objective], int(seg))

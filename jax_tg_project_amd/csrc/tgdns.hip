@@ -361,6 +361,27 @@ struct Ops {
     L.nblk = Launch<R, N>::fwd_x_blocks(0, true);
     return L;
   }
+  // `len` RK4 steps from L.cur keeping the four stage states of step j in
+  // L.st[4j .. 4j+3] (stage 1 = the step's input state).
+  static int steps_keeping_stages(const tg_plan* p, const AdjLayout& L, int len, double dt, double nu, void* ws,
+                                  hipStream_t s) {
+    Ptrs<R> w = carve<R>(p, ws);
+    const size_t bytes = 3 * p->compact * sizeof(cpx<R>);
+    for (int j = 0; j < len; ++j) {
+      cpx<R>* S = L.st + size_t(4 * j) * L.vstride;
+      TG_HIP(hipMemcpyAsync(S, L.cur, bytes, hipMemcpyDeviceToDevice, s));
+      for (int stg = 0; stg < 4; ++stg) {
+        StageArgs<R> a{};
+        a.u0 = L.cur;
+        a.us = S + size_t(stg) * L.vstride;
+        a.unext = stg == 3 ? L.cur : S + size_t(stg + 1) * L.vstride;
+        a.uacc = w.Acc;
+        a.dt = R(dt); a.nu = R(nu); a.ks = R(p->ks); a.stage = stg;
+        TG_RET(rhs_passes<R, N>(p, w, a.us, nullptr, FM_HYDRO_STAGE, a, R(0), s));
+      }
+    }
+    return TG_OK;
+  }
   // Forward sweep: nsteps RK4 steps from x0 with a checkpoint every `seg`
   // steps, J = E(T) (objective 0) or Z(T) (1), and the terminal cotangent.
   static int adj_forward(const tg_plan* p, const cpx<R>* x0, int nsteps, double dt, double nu, int objective,
@@ -370,7 +391,9 @@ struct Ops {
     TG_HIP(hipMemcpyAsync(L.cur, x0, bytes, hipMemcpyDeviceToDevice, s));
     for (int c = 0; c < L.nck; ++c) {
       TG_HIP(hipMemcpyAsync(L.ck + c * L.vstride, L.cur, bytes, hipMemcpyDeviceToDevice, s));
-      TG_RET(rk4(p, L.cur, nullptr, std::min(seg, nsteps - c * seg), dt, nu, 0.0, -1.0, ws, s));
+      const int len = std::min(seg, nsteps - c * seg);
+      if (c < L.nck - 1) TG_RET(rk4(p, L.cur, nullptr, len, dt, nu, 0.0, -1.0, ws, s));
+      else TG_RET(steps_keeping_stages(p, L, len, dt, nu, ws, s));  // the reverse sweep starts here
     }
     double d[NDIAG + 2];
     TG_RET(diagnostics(p, L.cur, nullptr, 0, d, ws, s));
@@ -397,19 +420,9 @@ struct Ops {
     const R cm[4] = {R(0), h / R(2), h / R(2), h};              // weight of mu_{s+1} in a_s
     for (int c = L.nck - 1; c >= 0; --c) {
       const int len = std::min(seg, nsteps - c * seg);
-      TG_HIP(hipMemcpyAsync(L.cur, L.ck + c * L.vstride, bytes, hipMemcpyDeviceToDevice, s));
-      for (int j = 0; j < len; ++j) {
-        cpx<R>* S = L.st + size_t(4 * j) * L.vstride;
-        TG_HIP(hipMemcpyAsync(S, L.cur, bytes, hipMemcpyDeviceToDevice, s));
-        for (int stg = 0; stg < 4; ++stg) {
-          StageArgs<R> a{};
-          a.u0 = L.cur;
-          a.us = S + size_t(stg) * L.vstride;
-          a.unext = stg == 3 ? L.cur : S + size_t(stg + 1) * L.vstride;
-          a.uacc = w.Acc;
-          a.dt = R(dt); a.nu = R(nu); a.ks = R(p->ks); a.stage = stg;
-          TG_RET(rhs_passes<R, N>(p, w, a.us, nullptr, FM_HYDRO_STAGE, a, R(0), s));
-        }
+      if (c < L.nck - 1) {  // the last segment's stages are kept by the forward sweep
+        TG_HIP(hipMemcpyAsync(L.cur, L.ck + c * L.vstride, bytes, hipMemcpyDeviceToDevice, s));
+        TG_RET(steps_keeping_stages(p, L, len, dt, nu, ws, s));
       }
       for (int j = len - 1; j >= 0; --j) {
         cpx<R>* S = L.st + size_t(4 * j) * L.vstride;
