@@ -173,6 +173,9 @@ def main():
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="only run a 1 GiB device copy (counter calibration) and 3 tuned FP64 steps, "
+                         "for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this pseudo-spectral path runs on one GPU; --gpus {args.gpus} is not supported "
@@ -182,6 +185,22 @@ def main():
     import torch
     from jax_tg_project_amd import roofline
     N, dt, nu = args.size, 2e-3, 1.0 / 1600.0
+    if args.traffic_probe:
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")  # 1 GiB
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        from jax_tg_project_amd._lib import Plan
+        from jax_tg_project_amd import autotune
+        plan = Plan(N, dtype=torch.float64)
+        tuned = autotune.tune(plan, use_cache=False)[0]
+        uh = plan.forward(tg_initial(N, torch.float64, plan.device))
+        plan.rk4(uh, nsteps=3, dt=dt, nu=nu)
+        torch.cuda.synchronize()
+        print(json.dumps({"traffic_probe": True, "copy_bytes_read": 1 << 30, "copy_bytes_written": 1 << 30,
+                          "tile_variants": tuned}), flush=True)
+        return
     K, W = args.steps, args.warmup
 
     r64 = run_gpu(N, torch.float64, K, W, dt, nu, tune=not args.no_tune)

@@ -260,7 +260,7 @@ __device__ __forceinline__ void io_stage(cpx<R>* buf, int LP, int l, int t, bool
       for (int m = 0; m < RAD; ++m) {
         const int n = b + m * SIN;
         if constexpr (SRC == 0) v[j][m] = buf[lidx<PS>(n, l, LP)];
-        else v[j][m] = ld(n, l);
+        else v[j][m] = ld(n, l, j * RAD + m);  // slot j*RAD+m is a compile-time constant
       }
     }
   }
@@ -318,7 +318,9 @@ __host__ __device__ constexpr int ilog2(int v) { return v <= 1 ? 0 : 1 + ilog2(v
 template <int N, int E>
 constexpr int pad_shift() { return ilog2(FirstRadix<typename PlanFor<N, E>::type>::value); }
 
-// Transform `nlines` lines whose first stage reads through `ld(n, l)` and
+// Transform `nlines` lines whose first stage reads through `ld(n, l, slot)`
+// (slot = the thread's register slot of that point, 0..E-1, fixed for a
+// given thread and n) and
 // (if DST_GLOBAL) whose last stage writes through `st(n, l, v)` in natural
 // order; intermediate stages exchange through LDS (buf[n*LP + l]).  The
 // caller guarantees the LDS buffer is free on entry (barrier after any

@@ -28,6 +28,16 @@ struct Geo {
   __device__ static constexpr int kint(int i) { return i <= M ? i : i - K; }
 };
 
+// Row pitch (complex elements) of the intermediate arrays A and B: KZ
+// rounded up to a whole number of 128-byte lines, so every tile of 8 (FP64)
+// or 16 (FP32) columns starts on a cache-line boundary.  The compact state
+// keeps the dense pitch KZ.
+template <typename R, int N>
+constexpr int kzp() {
+  constexpr int per_line = 128 / int(2 * sizeof(R));
+  return (Geo<N>::KZ + per_line - 1) / per_line * per_line;
+}
+
 // Launch shapes.  Grids of N >= 128 compile NVAR tile variants per pass;
 // the plan picks one per pass at run time (see the autotuner); smaller grids
 // compile variant 0 only.
@@ -112,13 +122,49 @@ __device__ __forceinline__ cpx<R> xfield(int f, const cpx<R>* __restrict__ su,
   return cmuli(r);
 }
 
+// xfield on register-held components: sv[c][slot] holds component c (0-2
+// the state, 3-5 the second field: b for MHD, the cotangent for ADJ).
+template <typename R, int N, int MODE, int NS, int E>
+__device__ __forceinline__ cpx<R> xfield_reg(int f, const cpx<R> (&sv)[NS][E], int slot, R kx, R ky, R kz,
+                                             R alpha) {
+  int o = 0;
+  if constexpr (MODE == XM_ADJ) {
+    if (f >= 6) {
+      const cpx<R> a = sv[3][slot], b = sv[4][slot], c = sv[5][slot];
+      const R k2 = kx * kx + ky * ky + kz * kz;
+      const R k2s = (k2 == R(0)) ? R(1) : k2;
+      const cpx<R> kd = {kx * a.x + ky * b.x + kz * c.x, kx * a.y + ky * b.y + kz * c.y};
+      const R kc = f == 6 ? kx : (f == 7 ? ky : kz);
+      const cpx<R> v = f == 6 ? a : (f == 7 ? b : c);
+      return {alpha * (v.x - kc * kd.x / k2s), alpha * (v.y - kc * kd.y / k2s)};
+    }
+  }
+  if constexpr (MODE == XM_MHD) {
+    if (f >= 6) { o = 3; f -= 6; }
+  }
+  if constexpr (MODE == XM_CURL3) f += 3;
+  const cpx<R> a = sv[o][slot], b = sv[o + 1][slot], c = sv[o + 2][slot];
+  if constexpr (MODE == XM_DIV) {
+    return cmuli(cpx<R>{kx * a.x + ky * b.x + kz * c.x, kx * a.y + ky * b.y + kz * c.y});
+  } else {
+    if (f == 0) return a;
+    if (f == 1) return b;
+    if (f == 2) return c;
+    cpx<R> r;
+    if (f == 3) r = {ky * c.x - kz * b.x, ky * c.y - kz * b.y};
+    else if (f == 4) r = {kz * a.x - kx * c.x, kz * a.y - kx * c.y};
+    else r = {kx * b.x - ky * a.x, kx * b.y - ky * a.y};
+    return cmuli(r);
+  }
+}
+
 template <typename R, int N, int MODE, int V>
 struct XInvBody {
   using G = Geo<N>;
   using C = CfgXY<R, N, V>;
   static constexpr int SMEM = N * C::L + N;  // line tile + twiddles
   // Block (bx, by) = (kz tile, iy).
-  __device__ static void run(cpx<R>* sm, int bx, int by, const cpx<R>* __restrict__ su,
+  __device__ __forceinline__ static void run(cpx<R>* sm, int bx, int by, const cpx<R>* __restrict__ su,
                              const cpx<R>* __restrict__ sb, cpx<R>* __restrict__ out,
                              const cpx<R>* __restrict__ tw, R ks, R alpha = R(1)) {
     constexpr int L = C::L, E = C::E;
@@ -130,22 +176,61 @@ struct XInvBody {
     const R scale = R(1) / (R(N) * R(N) * R(N));
     const R ky = ks * R(G::kint(iy));
     stage_twiddles<R, N>(tws, tw);
+    // The first FFT stage of every field reads the same E points per thread:
+    // load the state (and cotangent) components at those points once.
+    constexpr int NS = (MODE == XM_HYDRO || MODE == XM_CURL3 || MODE == XM_DIV || MODE == XM_COPY3) ? 3
+                     : (MODE == XM_MHD || MODE == XM_ADJ) ? 6 : 3;
+    constexpr int TPL = N / E;
+    const int tid = threadIdx.x;
+    const int l = tid % L, t = tid / L;
+    const bool act = t < TPL;
+    const int izr = iz0 + l;
+    const bool colok = izr < G::KZ;
+    const int iz = colok ? izr : G::KZ - 1;  // clamp inside the row: same cache line
+    // Registers only while the preload stays small; otherwise every field
+    // reads its components from global memory (cache-served re-reads).
+    constexpr bool PRE = NS * E <= 48;
+    cpx<R> sv[PRE ? NS : 1][PRE ? E : 1];
+    if constexpr (PRE) {
+      using Pl = typename PlanFor<N, E>::type;
+      constexpr int RAD0 = FirstRadix<Pl>::value;
+      constexpr int NB0 = E / RAD0, SIN0 = N / RAD0;
+#pragma unroll
+      for (int j = 0; j < NB0; ++j) {
+#pragma unroll
+        for (int m = 0; m < RAD0; ++m) {
+          const int n = (act ? t : 0) * NB0 + j + m * SIN0;
+          const bool ok = G::kept(n);
+          const int ix = ok ? G::compact_of(n) : G::M;
+          const size_t pt = (size_t(ix) * G::K + iy) * G::KZ + iz;
+#pragma unroll
+          for (int c = 0; c < NS; ++c) {
+            const cpx<R>* src = c < 3 ? su : sb;
+            sv[c][j * RAD0 + m] = src[(c % 3) * cs + pt];
+          }
+        }
+      }
+    }
+#pragma unroll 1
     for (int f = 0; f < F; ++f) {
       if (f) __syncthreads();  // the previous field's last LDS reads are done
-      auto ld = [&](int n, int ll) -> cpx<R> {
-        // branch-free: clamp to a valid address, load, then select zero
-        const int iz0c = iz0 + ll;
-        const bool ok = iz0c < G::KZ && G::kept(n);
-        const int iz = ok ? iz0c : 0;
+      auto ld = [&](int n, int ll, int slot) -> cpx<R> {
+        const bool ok = colok && G::kept(n);
         const int ix = ok ? G::compact_of(n) : 0;
-        const size_t pt = (size_t(ix) * G::K + iy) * G::KZ + iz;
-        const cpx<R> v =
-            cscale(xfield<R, N, MODE>(f, su, sb, pt, cs, ks * R(G::kint(ix)), ky, ks * R(iz), alpha), scale);
+        const R kx = ks * R(G::kint(ix)), kz = ks * R(iz);
+        cpx<R> v;
+        if constexpr (PRE) {
+          v = xfield_reg<R, N, MODE, NS, E>(f, sv, slot, kx, ky, kz, alpha);
+        } else {
+          const size_t pt = (size_t(ix) * G::K + iy) * G::KZ + iz;
+          v = xfield<R, N, MODE>(f, su, sb, pt, cs, kx, ky, kz, alpha);
+        }
+        v = cscale(v, scale);
         return ok ? v : cpx<R>{R(0), R(0)};
       };
       auto st = [&](int n, int ll, cpx<R> v) {
-        const int iz = iz0 + ll;
-        if (iz < G::KZ) out[((size_t(f) * N + n) * G::K + iy) * G::KZ + iz] = v;
+        const int izw = iz0 + ll;
+        if (izw < G::KZ) out[((size_t(f) * N + n) * G::K + iy) * kzp<R, N>() + izw] = v;
       };
       fft_io<R, N, E, +1, 1, true>(buf, L, L, tws, ld, st);
     }
@@ -172,7 +257,7 @@ struct YBody {
   using C = CfgXY<R, N, V>;
   static constexpr int SMEM = N * C::L + N;
   // Block (bx, by, bz) = (kz tile, x, field).
-  __device__ static void run(cpx<R>* sm, int bx, int by, int bz, const cpx<R>* __restrict__ in,
+  __device__ __forceinline__ static void run(cpx<R>* sm, int bx, int by, int bz, const cpx<R>* __restrict__ in,
                              cpx<R>* __restrict__ out, const cpx<R>* __restrict__ tw) {
     constexpr int L = C::L, E = C::E;
     cpx<R>* buf = sm;
@@ -180,16 +265,16 @@ struct YBody {
     const int iz0 = bx * L;
     const size_t plane = size_t(bz) * N + by;
     stage_twiddles<R, N>(tws, tw);
-    auto ld = [&](int n, int ll) -> cpx<R> {
+    auto ld = [&](int n, int ll, int) -> cpx<R> {
       const int izr = iz0 + ll;
       bool ok = izr < G::KZ;
-      const int iz = ok ? izr : 0;
+      const int iz = ok ? izr : G::KZ - 1;
       cpx<R> v;
       if constexpr (DIR > 0) {
         ok = ok && G::kept(n);
-        v = in[(plane * G::K + (ok ? G::compact_of(n) : 0)) * G::KZ + iz];
+        v = in[(plane * G::K + (ok ? G::compact_of(n) : 0)) * kzp<R, N>() + iz];
       } else {
-        v = in[(plane * N + n) * G::KZ + iz];
+        v = in[(plane * N + n) * kzp<R, N>() + iz];
       }
       return ok ? v : cpx<R>{R(0), R(0)};
     };
@@ -197,9 +282,9 @@ struct YBody {
       const int iz = iz0 + ll;
       if (iz >= G::KZ) return;
       if constexpr (DIR > 0) {
-        out[(plane * N + n) * G::KZ + iz] = v;
+        out[(plane * N + n) * kzp<R, N>() + iz] = v;
       } else {
-        if (G::kept(n)) out[(plane * G::K + G::compact_of(n)) * G::KZ + iz] = v;
+        if (G::kept(n)) out[(plane * G::K + G::compact_of(n)) * kzp<R, N>() + iz] = v;
       }
     };
     fft_io<R, N, E, DIR, 1, true>(buf, L, L, tws, ld, st);
@@ -261,7 +346,7 @@ struct ZBody {
   static constexpr int SMEM = BUF + N;
   // Block (bx, by) = (group of LZ y lines, x).  Threads at or beyond Z::NT
   // (a batched launch with a larger block) only join the barriers.
-  __device__ static void run(cpx<R>* sm, int bx, int by, const cpx<R>* __restrict__ in,
+  __device__ __forceinline__ static void run(cpx<R>* sm, int bx, int by, const cpx<R>* __restrict__ in,
                              cpx<R>* __restrict__ out, const R* __restrict__ phys_in,
                              R* __restrict__ phys_out, const cpx<R>* __restrict__ tw, R lorentz) {
   constexpr int NSEQ = Z::NSEQ, LP = Z::LP, NT = Z::NT, E = Z::E;
@@ -277,14 +362,15 @@ struct ZBody {
   const int tid = threadIdx.x;
   const bool live = tid < NT;
   const int x = by, y0 = bx * LZ;
-  const size_t NNKZ = size_t(N) * N * KZ;
+  constexpr int KZP = kzp<R, N>();  // pitch of the A/B rows
+  const size_t NNKZ = size_t(N) * N * KZP;
   const size_t line0 = size_t(x) * N + y0;  // (x, y0) line index
 
   if constexpr (MODE == ZM_ADJ) {
     // ---- inverse of u (f 0-2), curl u (3-5), q (6-8) for line pairs: per
     // pair g, sequences j = 0..3 (line 2g), 4..7 (line 2g+1) pack (u0,u1),
     // (u2,w0), (w1,w2), (q0,q1); j = 8 packs (q2 of 2g, q2 of 2g+1).
-    auto ld = [&](int k, int s) -> cpx<R> {
+    auto ld = [&](int k, int s, int) -> cpx<R> {
       const int g = s / 9, j = s % 9;
       int la, lb, fa, fb;
       if (j < 8) {
@@ -298,8 +384,8 @@ struct ZBody {
       const int kzr = neg ? N - k : k;
       const bool ok = kzr < KZ;
       const int kz = ok ? kzr : 0;
-      cpx<R> a = in[fa * NNKZ + (line0 + la) * KZ + kz];
-      cpx<R> b = in[fb * NNKZ + (line0 + lb) * KZ + kz];
+      cpx<R> a = in[fa * NNKZ + (line0 + la) * KZP + kz];
+      cpx<R> b = in[fb * NNKZ + (line0 + lb) * KZP + kz];
       if (k == 0) { a.y = R(0); b.y = R(0); }
       if (neg) { a.y = -a.y; b.y = -b.y; }
       const cpx<R> z = {a.x - b.y, a.y + b.x};
@@ -352,15 +438,15 @@ struct ZBody {
       const cpx<R> re = {R(0.5) * (P.x + Qc.x), R(0.5) * (P.y - Qc.y)};
       const cpx<R> im = {R(0.5) * (P.y + Qc.y), R(0.5) * (Qc.x - P.x)};
       const int line = q / 3, p = q % 3;
-      out[(2 * p) * NNKZ + (line0 + line) * KZ + kz] = re;
-      out[(2 * p + 1) * NNKZ + (line0 + line) * KZ + kz] = im;
+      out[(2 * p) * NNKZ + (line0 + line) * KZP + kz] = re;
+      out[(2 * p + 1) * NNKZ + (line0 + line) * KZP + kz] = im;
     }
     return;
   }
   if constexpr (MODE != ZM_PHYS_IN3) {
     // ---- inverse: sequence s = line*SPL + p packs fields (fa, fb) as a + i b;
     // the first FFT stage reads the Hermitian-extended coefficients directly.
-    auto ld = [&](int k, int s) -> cpx<R> {
+    auto ld = [&](int k, int s, int) -> cpx<R> {
       const int line = s / S::SPL, p = s % S::SPL;
       int fa, fb;
       if constexpr (MODE == ZM_HYDRO) { fa = p; fb = p + 3; }
@@ -371,7 +457,7 @@ struct ZBody {
       const int kzr = neg ? N - k : k;
       const bool ok = kzr < KZ;
       const int kz = ok ? kzr : 0;  // branch-free: clamped load, then select
-      const size_t off = (line0 + line) * KZ + kz;
+      const size_t off = (line0 + line) * KZP + kz;
       cpx<R> a = in[fa * NNKZ + off];
       cpx<R> b = in[(fb >= 0 ? fb : fa) * NNKZ + off];
       if (fb < 0) b = {R(0), R(0)};
@@ -498,8 +584,8 @@ struct ZBody {
         if (p < 2) { line_a = line_b = 2 * g + p; fa = 0; fb = 1; }
         else { line_a = 2 * g; line_b = 2 * g + 1; fa = 2; fb = 2; }
       }
-      out[fa * NNKZ + (line0 + line_a) * KZ + kz] = re;
-      out[fb * NNKZ + (line0 + line_b) * KZ + kz] = im;
+      out[fa * NNKZ + (line0 + line_a) * KZP + kz] = re;
+      out[fb * NNKZ + (line0 + line_b) * KZP + kz] = im;
     }
   }
   }
@@ -586,7 +672,7 @@ struct XFBody {
   static constexpr int LINES = CfgXF<R, N, V>::LINES;
   static constexpr int SMEM = N * LINES + N;
   // Block (bx, by) = (kz tile, iy).
-  __device__ static void run(cpx<R>* sm, int bx, int by, const cpx<R>* __restrict__ in, const StageArgs<R>& a,
+  __device__ __forceinline__ static void run(cpx<R>* sm, int bx, int by, const cpx<R>* __restrict__ in, const StageArgs<R>& a,
                              const cpx<R>* __restrict__ tw) {
   constexpr int FF = (MODE == FM_MHD_STAGE || MODE == FM_MHD_RHS || MODE == FM_ADJ) ? 6 : 3;
   constexpr int NT = CfgXF<R, N, V>::NT, E = CfgXF<R, N, V>::E;
@@ -598,10 +684,10 @@ struct XFBody {
   const int iz0 = bx * L, iy = by;
   const int tid = threadIdx.x;
   stage_twiddles<R, N>(tws, tw);
-  auto ld = [&](int n, int ln) -> cpx<R> {
+  auto ld = [&](int n, int ln, int) -> cpx<R> {
     const int f = ln / L, izr = iz0 + ln % L;
     const bool ok = izr < G::KZ;
-    const cpx<R> v = in[((size_t(f) * N + n) * G::K + iy) * G::KZ + (ok ? izr : 0)];
+    const cpx<R> v = in[((size_t(f) * N + n) * G::K + iy) * kzp<R, N>() + (ok ? izr : G::KZ - 1)];
     return ok ? v : cpx<R>{R(0), R(0)};
   };
   fft_io<R, N, E, -1, 1, false>(buf, LP, LINES, tws, ld, NoIO{});

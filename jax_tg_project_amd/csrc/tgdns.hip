@@ -529,6 +529,23 @@ int check_plan(const tg_plan* p) {
 template <int N>
 int nmax_for() { return Launch<float, N>::nmax(); }
 
+// Padded row pitch of the intermediate arrays for a run-time N.
+template <typename R>
+int kzp_rt(int n) {
+  switch (n) {
+    case 4: return kzp<R, 4>();
+    case 8: return kzp<R, 8>();
+    case 16: return kzp<R, 16>();
+    case 32: return kzp<R, 32>();
+    case 48: return kzp<R, 48>();
+    case 64: return kzp<R, 64>();
+    case 96: return kzp<R, 96>();
+    case 128: return kzp<R, 128>();
+    case 256: return kzp<R, 256>();
+  }
+  return 0;
+}
+
 int nmax_rt(int n) {
   switch (n) {
     case 4: return nmax_for<4>();
@@ -580,8 +597,9 @@ int tg_plan_create(int n, int precision_bits, int mhd, double domain_length, tg_
   const size_t fi = p->mhd ? 12 : 9;  // inverse fields: hydro RHS 6, adjoint 9, MHD 12
   const size_t fo = 6;                 // z-pass outputs: hydro 3, adjoint and MHD 6
   // A holds the inverse-x output (fi x N K KZ) and later the z output (fo x N N KZ)
-  const size_t sA = std::max(fi * N * p->K, fo * N * N) * p->KZ * p->cb;
-  const size_t sB = fi * N * N * p->KZ * p->cb;
+  const size_t kzpad = precision_bits == 64 ? size_t(kzp_rt<double>(n)) : size_t(kzp_rt<float>(n));
+  const size_t sA = std::max(fi * N * p->K, fo * N * N) * kzpad * p->cb;
+  const size_t sB = fi * N * N * kzpad * p->cb;
   const size_t sU = (p->mhd ? 2 : 1) * 3 * p->compact * p->cb;
   p->nDiagBlk = 512;
   p->nMax = nmax_rt(n);
@@ -885,7 +903,7 @@ int batch_bind(tg_batch* b, void* state, void* ws) {
     const int i = b->order[slot];
     const int N = b->n[i], M = N / 3, K = 2 * M + 1, KZ = M + 1;
     const size_t vec = size_t(3) * K * K * KZ * sizeof(cpx<R>);
-    const size_t sA = size_t(6) * N * K * KZ * sizeof(cpx<R>);
+    const size_t sA = size_t(6) * N * K * kzp_rt<R>(N) * sizeof(cpx<R>);
     char* base = w + b->ws_off[i];
     EnsInst<R>& I = h[slot];
     I.n = N;
@@ -932,8 +950,9 @@ int tg_batch_create(int ninst, const int* n, const double* nu, int precision_bit
     const size_t N = n[i], M = N / 3, K = 2 * M + 1, KZ = M + 1;
     const size_t vec = 3 * K * K * KZ * b->cb;
     b->state_off[i + 1] = b->state_off[i] + (long long)(3 * K * K * KZ);
-    b->ws_off[i + 1] = b->ws_off[i] + 2 * align256(vec) + align256(6 * N * K * KZ * b->cb) +
-                       align256(6 * N * N * KZ * b->cb);
+    const size_t kzpad = precision_bits == 64 ? size_t(kzp_rt<double>(n[i])) : size_t(kzp_rt<float>(n[i]));
+    b->ws_off[i + 1] = b->ws_off[i] + 2 * align256(vec) + align256(6 * N * K * kzpad * b->cb) +
+                       align256(6 * N * N * kzpad * b->cb);
   }
   for (int k = 0; k < kNumEnsSizes; ++k) {
     int cnt = 0;
