@@ -23,6 +23,26 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hydro_256_fp64_traffic.txt")
+KERNEL_OF_PASS = {"inv_x": "k_inv_x<double, 256, 0, {v}>", "inv_y": "k_y<double, 256, 1, {v}>",
+                  "z": "k_z<double, 256, 0, {v}>", "fwd_y": "k_y<double, 256, -1, {v}>",
+                  "fwd_x": "k_fwd_x<double, 256, 0, {v}>"}
+PASS_INDEX = {"inv_x": 0, "inv_y": 1, "z": 2, "fwd_y": 3, "fwd_x": 4}
+
+
+def measured_traffic(pass_name, variants):
+    """HBM bytes per launch of the FP64 256^3 kernel for this pass and tile
+    variant, from the committed rocprofv3 counter summary (None if absent)."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            rows = [ln.rstrip("\n").split("\t") for ln in f if not ln.startswith("#")]
+    except OSError:
+        return None
+    want = "tg::" + KERNEL_OF_PASS[pass_name].format(v=variants[PASS_INDEX[pass_name]])
+    for r in rows:
+        if len(r) == 3 and r[0] == want:
+            return float(r[1]) + float(r[2])
+    return None
 
 
 def cpu_model():
@@ -234,7 +254,8 @@ def main():
                                f"Leray projection, {K} steps", "N": N, "nu": nu, "dt": dt},
         "hbm_GBps_algorithmic_step": step_b / (r64["wall"] / K) / 1e9,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(dom, r64["variants"]) if N == 256 else None,
                      "bytes_per_launch": dom_bytes, "ms_per_launch": dom_ms_avg},
         "passes_fp64": per_pass,
         "tile_variants_fp64": r64["variants"],
