@@ -164,15 +164,16 @@ struct XInvBody {
   using C = CfgXY<R, N, V>;
   static constexpr int SMEM = N * C::L + N;  // line tile + twiddles
   // Block (bx, by) = (kz tile, iy).
+  // cpz: row pitch (complex elements) of the compact state arrays su, sb.
   __device__ __forceinline__ static void run(cpx<R>* sm, int bx, int by, const cpx<R>* __restrict__ su,
                              const cpx<R>* __restrict__ sb, cpx<R>* __restrict__ out,
-                             const cpx<R>* __restrict__ tw, R ks, R alpha = R(1)) {
+                             const cpx<R>* __restrict__ tw, R ks, R alpha = R(1), int cpz = Geo<N>::KZ) {
     constexpr int L = C::L, E = C::E;
     constexpr int F = xmode_fields(MODE);
     cpx<R>* buf = sm;
     cpx<R>* tws = sm + N * L;
     const int iz0 = bx * L, iy = by;
-    const size_t cs = size_t(G::K) * G::K * G::KZ;
+    const size_t cs = size_t(G::K) * G::K * cpz;
     const R scale = R(1) / (R(N) * R(N) * R(N));
     const R ky = ks * R(G::kint(iy));
     stage_twiddles<R, N>(tws, tw);
@@ -202,7 +203,7 @@ struct XInvBody {
           const int n = (act ? t : 0) * NB0 + j + m * SIN0;
           const bool ok = G::kept(n);
           const int ix = ok ? G::compact_of(n) : G::M;
-          const size_t pt = (size_t(ix) * G::K + iy) * G::KZ + iz;
+          const size_t pt = (size_t(ix) * G::K + iy) * cpz + iz;
 #pragma unroll
           for (int c = 0; c < NS; ++c) {
             const cpx<R>* src = c < 3 ? su : sb;
@@ -222,7 +223,7 @@ struct XInvBody {
         if constexpr (PRE) {
           v = xfield_reg<R, N, MODE, NS, E>(f, sv, slot, kx, ky, kz, alpha);
         } else {
-          const size_t pt = (size_t(ix) * G::K + iy) * G::KZ + iz;
+          const size_t pt = (size_t(ix) * G::K + iy) * cpz + iz;
           v = xfield<R, N, MODE>(f, su, sb, pt, cs, kx, ky, kz, alpha);
         }
         v = cscale(v, scale);
@@ -242,10 +243,10 @@ __global__ __launch_bounds__((CfgXY<R, N, V>::NT)) void k_inv_x(const cpx<R>* __
                                                               const cpx<R>* __restrict__ sb,
                                                               cpx<R>* __restrict__ out,
                                                               const cpx<R>* __restrict__ tw, R ks,
-                                                              R alpha) {
+                                                              R alpha, int cpz) {
   using B = XInvBody<R, N, MODE, V>;
   __shared__ cpx<R> sm[B::SMEM];
-  B::run(sm, blockIdx.x, blockIdx.y, su, sb, out, tw, ks, alpha);
+  B::run(sm, blockIdx.x, blockIdx.y, su, sb, out, tw, ks, alpha, cpz);
 }
 
 // ----------------------------------------------------------------- pass 2/4
@@ -619,6 +620,7 @@ struct StageArgs {
   cpx<R>* bacc;
   R dt, nu, eta, ks;
   int stage;
+  int cpz;  // row pitch of the compact arrays (0 = dense KZ)
   // adjoint (FM_ADJ): mu = P-less VJP of the RHS at the stage state xs for
   // the cotangent a_s = alpha * a;  lacc <- (lamp | lacc | 0) + mu by
   // `first` (1 | 0 | 2);  anext <- clam * lamp + cmu * mu unless `last`;
@@ -691,13 +693,14 @@ struct XFBody {
     return ok ? v : cpx<R>{R(0), R(0)};
   };
   fft_io<R, N, E, -1, 1, false>(buf, LP, LINES, tws, ld, NoIO{});
-  const size_t cs = size_t(G::K) * G::K * G::KZ;
+  const int cpz = a.cpz ? a.cpz : G::KZ;
+  const size_t cs = size_t(G::K) * G::K * cpz;
   const R ky = a.ks * R(G::kint(iy));
   for (int idx = tid; tid < NT && idx < G::K * L; idx += NT) {
     const int ll = idx % L, ix = idx / L, iz = iz0 + ll;
     if (iz >= G::KZ) continue;
     const int n = G::full_of(ix);
-    const size_t pt = (size_t(ix) * G::K + iy) * G::KZ + iz;
+    const size_t pt = (size_t(ix) * G::K + iy) * cpz + iz;
     cpx<R> v[FF];
 #pragma unroll
     for (int f = 0; f < FF; ++f) v[f] = buf[n * LP + f * L + ll];

@@ -53,7 +53,7 @@ struct tg_plan {
   size_t cb;          // bytes per complex element
   size_t compact;     // complex elements per compact component
   // workspace layout (byte offsets)
-  size_t offA, offB, offU1, offAcc, offPart, offMax, offOut, wsBytes;
+  size_t offA, offB, offU0, offU1, offAcc, offPart, offMax, offOut, wsBytes;
   int nDiagBlk, nMax;
   int var[TG_NPASS] = {0, 0, 0, 0, 0};  // tile variant per pass
   // optional per-pass timing: pairs of events around each pass launch
@@ -69,6 +69,7 @@ template <typename R>
 struct Ptrs {
   cpx<R>* A;
   cpx<R>* B;
+  cpx<R>* U0;    // padded step state u (and b right after it for MHD)
   cpx<R>* U1;    // stage input u (and b right after it for MHD)
   cpx<R>* Acc;   // accumulator u (and b)
   double* part;
@@ -83,6 +84,7 @@ Ptrs<R> carve(const tg_plan* p, void* ws) {
   Ptrs<R> q;
   q.A = reinterpret_cast<cpx<R>*>(w + p->offA);
   q.B = reinterpret_cast<cpx<R>*>(w + p->offB);
+  q.U0 = reinterpret_cast<cpx<R>*>(w + p->offU0);
   q.U1 = reinterpret_cast<cpx<R>*>(w + p->offU1);
   q.Acc = reinterpret_cast<cpx<R>*>(w + p->offAcc);
   q.part = reinterpret_cast<double*>(w + p->offPart);
@@ -116,18 +118,18 @@ template <typename R, int N>
 struct Launch {
   using G = Geo<N>;
   static int inv_x(int mode, int var, const cpx<R>* su, const cpx<R>* sb, cpx<R>* out, const cpx<R>* tw, R ks,
-                   hipStream_t s, R al = R(1)) {
+                   hipStream_t s, R al = R(1), int cpz = Geo<N>::KZ) {
     return with_variant<N>(var, [&](auto vc) {
       constexpr int V = decltype(vc)::value;
       using C = CfgXY<R, N, V>;
       dim3 grid(cdiv(G::KZ, C::L), G::K), blk(C::NT);
       switch (mode) {
-        case XM_HYDRO: hipLaunchKernelGGL((k_inv_x<R, N, XM_HYDRO, V>), grid, blk, 0, s, su, sb, out, tw, ks, al); break;
-        case XM_MHD: hipLaunchKernelGGL((k_inv_x<R, N, XM_MHD, V>), grid, blk, 0, s, su, sb, out, tw, ks, al); break;
-        case XM_COPY3: hipLaunchKernelGGL((k_inv_x<R, N, XM_COPY3, V>), grid, blk, 0, s, su, sb, out, tw, ks, al); break;
-        case XM_CURL3: hipLaunchKernelGGL((k_inv_x<R, N, XM_CURL3, V>), grid, blk, 0, s, su, sb, out, tw, ks, al); break;
-        case XM_ADJ: hipLaunchKernelGGL((k_inv_x<R, N, XM_ADJ, V>), grid, blk, 0, s, su, sb, out, tw, ks, al); break;
-        default: hipLaunchKernelGGL((k_inv_x<R, N, XM_DIV, V>), grid, blk, 0, s, su, sb, out, tw, ks, al); break;
+        case XM_HYDRO: hipLaunchKernelGGL((k_inv_x<R, N, XM_HYDRO, V>), grid, blk, 0, s, su, sb, out, tw, ks, al, cpz); break;
+        case XM_MHD: hipLaunchKernelGGL((k_inv_x<R, N, XM_MHD, V>), grid, blk, 0, s, su, sb, out, tw, ks, al, cpz); break;
+        case XM_COPY3: hipLaunchKernelGGL((k_inv_x<R, N, XM_COPY3, V>), grid, blk, 0, s, su, sb, out, tw, ks, al, cpz); break;
+        case XM_CURL3: hipLaunchKernelGGL((k_inv_x<R, N, XM_CURL3, V>), grid, blk, 0, s, su, sb, out, tw, ks, al, cpz); break;
+        case XM_ADJ: hipLaunchKernelGGL((k_inv_x<R, N, XM_ADJ, V>), grid, blk, 0, s, su, sb, out, tw, ks, al, cpz); break;
+        default: hipLaunchKernelGGL((k_inv_x<R, N, XM_DIV, V>), grid, blk, 0, s, su, sb, out, tw, ks, al, cpz); break;
       }
       TG_LAUNCHED();
       return TG_OK;
@@ -249,12 +251,14 @@ int timed_end(const tg_plan* cp, hipStream_t s) {
 template <typename R, int N>
 int rhs_passes(const tg_plan* p, const Ptrs<R>& w, const cpx<R>* us, const cpx<R>* bs, int fmode,
                const StageArgs<R>& a, R lor, hipStream_t s) {
+  const int cpz = a.cpz ? a.cpz : Geo<N>::KZ;
   using Lc = Launch<R, N>;
   const cpx<R>* tw = static_cast<const cpx<R>*>(p->tw);
   const bool mhd = p->mhd != 0;
   const int fi = mhd ? 12 : 6, fo = mhd ? 6 : 3;
   const int* v = p->var;
-  TG_TIMED(TG_PASS_INV_X, Lc::inv_x(mhd ? XM_MHD : XM_HYDRO, v[TG_PASS_INV_X], us, bs, w.A, tw, R(p->ks), s));
+  TG_TIMED(TG_PASS_INV_X, Lc::inv_x(mhd ? XM_MHD : XM_HYDRO, v[TG_PASS_INV_X], us, bs, w.A, tw, R(p->ks), s, R(1),
+                                    cpz));
   TG_TIMED(TG_PASS_INV_Y, Lc::y(+1, v[TG_PASS_INV_Y], fi, w.A, w.B, tw, s));
   TG_TIMED(TG_PASS_Z, Lc::z(mhd ? ZM_MHD : ZM_HYDRO, v[TG_PASS_Z], w.B, w.A, nullptr, nullptr, tw, lor, s));
   TG_TIMED(TG_PASS_FWD_Y, Lc::y(-1, v[TG_PASS_FWD_Y], fo, w.A, w.B, tw, s));
@@ -293,29 +297,44 @@ struct Ops {
     a.nu = R(nu); a.eta = R(eta); a.ks = R(p->ks);
     return rhs_passes<R, N>(p, w, u, b, p->mhd ? FM_MHD_RHS : FM_HYDRO_RHS, a, R(lor), s);
   }
+  // nsteps RK4 steps.  The loop runs on line-padded internal copies of the
+  // state (row pitch kzp instead of the dense KZ of the ABI layout), copied
+  // in once and out once per call.
   static int rk4(const tg_plan* p, cpx<R>* u, cpx<R>* b, int nsteps, double dt, double nu, double eta,
                  double lor, void* ws, hipStream_t s) {
+    if (nsteps == 0) return TG_OK;
     Ptrs<R> w = carve<R>(p, ws);
-    const size_t vec = 3 * p->compact;
+    constexpr int KZP = kzp<R, N>();
+    constexpr int KZ = Geo<N>::KZ;
+    const size_t rows = size_t(3) * p->K * p->K;
+    const size_t vec = rows * KZP;  // padded vector (3 components)
+    cpx<R>* u0 = w.U0;
+    cpx<R>* b0 = w.U0 + vec;
     cpx<R>* u1 = w.U1;
     cpx<R>* b1 = w.U1 + vec;
     cpx<R>* ua = w.Acc;
     cpx<R>* ba = w.Acc + vec;
+    const size_t e = sizeof(cpx<R>);
+    TG_HIP(hipMemcpy2DAsync(u0, KZP * e, u, KZ * e, KZ * e, rows, hipMemcpyDeviceToDevice, s));
+    if (p->mhd) TG_HIP(hipMemcpy2DAsync(b0, KZP * e, b, KZ * e, KZ * e, rows, hipMemcpyDeviceToDevice, s));
     const int fm = p->mhd ? FM_MHD_STAGE : FM_HYDRO_STAGE;
     for (int step = 0; step < nsteps; ++step) {
       for (int st = 0; st < 4; ++st) {
         StageArgs<R> a{};
-        a.u0 = u; a.b0 = b;
-        a.us = st == 0 ? u : u1;
-        a.bs = st == 0 ? b : b1;
-        a.unext = st == 3 ? u : u1;
-        a.bnext = st == 3 ? b : b1;
+        a.u0 = u0; a.b0 = b0;
+        a.us = st == 0 ? u0 : u1;
+        a.bs = st == 0 ? b0 : b1;
+        a.unext = st == 3 ? u0 : u1;
+        a.bnext = st == 3 ? b0 : b1;
         a.uacc = ua; a.bacc = ba;
         a.dt = R(dt); a.nu = R(nu); a.eta = R(eta); a.ks = R(p->ks);
         a.stage = st;
+        a.cpz = KZP;
         TG_RET(rhs_passes<R, N>(p, w, a.us, a.bs, fm, a, R(lor), s));
       }
     }
+    TG_HIP(hipMemcpy2DAsync(u, KZ * e, u0, KZP * e, KZ * e, rows, hipMemcpyDeviceToDevice, s));
+    if (p->mhd) TG_HIP(hipMemcpy2DAsync(b, KZ * e, b0, KZP * e, KZ * e, rows, hipMemcpyDeviceToDevice, s));
     return TG_OK;
   }
   // One vector-Jacobian product of the hydro RHS at x for the cotangent a
@@ -600,12 +619,14 @@ int tg_plan_create(int n, int precision_bits, int mhd, double domain_length, tg_
   const size_t kzpad = precision_bits == 64 ? size_t(kzp_rt<double>(n)) : size_t(kzp_rt<float>(n));
   const size_t sA = std::max(fi * N * p->K, fo * N * N) * kzpad * p->cb;
   const size_t sB = fi * N * N * kzpad * p->cb;
-  const size_t sU = (p->mhd ? 2 : 1) * 3 * p->compact * p->cb;
+  // padded state vectors (row pitch kzpad) for the RK4 loop
+  const size_t sU = (p->mhd ? 2 : 1) * 3 * size_t(p->K) * p->K * kzpad * p->cb;
   p->nDiagBlk = 512;
   p->nMax = nmax_rt(n);
   size_t o = 0;
   p->offA = o; o += align256(sA);
   p->offB = o; o += align256(sB);
+  p->offU0 = o; o += align256(sU);
   p->offU1 = o; o += align256(sU);
   p->offAcc = o; o += align256(sU);
   p->offPart = o; o += align256(size_t(p->nDiagBlk) * NDIAG * sizeof(double));
