@@ -43,10 +43,16 @@ struct EnsGrid {
   static constexpr int GX_XY = cdiv_c(G::KZ, XY::L);
   static constexpr int LZ = CfgZM<R, N, ZM_HYDRO, 0>::LZ;
   static constexpr int GX_XF = cdiv_c(G::KZ, XF::LINES / 3);
+  static constexpr int ZE = ZwDefault<N>::E, ZWPB = ZwDefault<N>::WPB;
+  static constexpr int zlines() {
+    if constexpr (ZE > 0) return CfgZW<R, N, ZE, ZWPB>::LINES;
+    else return LZ;
+  }
+  static constexpr int ZBLOCKS = N * N / zlines();
   static constexpr int blocks(int pass) {
     return pass == EP_INV_X ? GX_XY * G::K
          : pass == EP_INV_Y ? GX_XY * N * 6
-         : pass == EP_Z ? (N / LZ) * N
+         : pass == EP_Z ? ZBLOCKS
          : pass == EP_FWD_Y ? GX_XY * N * 3
          : GX_XF * G::K;
   }
@@ -63,8 +69,12 @@ __device__ __forceinline__ void ens_run(cpx<R>* sm, int lb, const EnsInst<R>& I,
     if constexpr (PASS == EP_INV_Y) YBody<R, N, +1, 0>::run(sm, bx, rest % N, rest / N, I.A, I.B, I.tw);
     else YBody<R, N, -1, 0>::run(sm, bx, rest % N, rest / N, I.A, I.B, I.tw);
   } else if constexpr (PASS == EP_Z) {
-    constexpr int gx = N / Gd::LZ;
-    ZBody<R, N, ZM_HYDRO, 0>::run(sm, lb % gx, lb / gx, I.B, I.A, nullptr, nullptr, I.tw, R(0));
+    if constexpr (Gd::ZE > 0) {
+      ZWBody<R, N, Gd::ZE, Gd::ZWPB>::run(sm, lb, I.B, I.A, I.tw);
+    } else {
+      constexpr int gx = N / Gd::LZ;
+      ZBody<R, N, ZM_HYDRO, 0>::run(sm, lb % gx, lb / gx, I.B, I.A, nullptr, nullptr, I.tw, R(0));
+    }
   } else {
     StageArgs<R> a{};
     a.u0 = I.u0;
@@ -79,21 +89,36 @@ __device__ __forceinline__ void ens_run(cpx<R>* sm, int lb, const EnsInst<R>& I,
   }
 }
 
+template <typename R, int N, bool ZW = (ZwDefault<N>::E > 0)>
+struct EnsZ {  // LDS-resident z body
+  static constexpr int NT = CfgZM<R, N, ZM_HYDRO, 0>::NT;
+  static constexpr int SMEM = ZBody<R, N, ZM_HYDRO, 0>::SMEM;
+  static constexpr int WAVES = 1;
+};
+template <typename R, int N>
+struct EnsZ<R, N, true> {  // register-resident z body
+  static constexpr int NT = 64 * ZwDefault<N>::WPB;
+  static constexpr int SMEM = ZWBody<R, N, ZwDefault<N>::E, ZwDefault<N>::WPB>::SMEM;
+  static constexpr int WAVES = zw_waves_per_eu<R, ZwDefault<N>::E>();
+};
+
 template <typename R, int N, int PASS>
 struct EnsBlock {
-  static constexpr int NT = PASS == EP_Z ? CfgZM<R, N, ZM_HYDRO, 0>::NT
+  static constexpr int NT = PASS == EP_Z ? EnsZ<R, N>::NT
                           : PASS == EP_FWD_X ? CfgXF<R, N, 0>::NT : CfgXY<R, N, 0>::NT;
   static constexpr int SMEM = PASS == EP_INV_X ? XInvBody<R, N, XM_HYDRO, 0>::SMEM
-                            : PASS == EP_Z ? ZBody<R, N, ZM_HYDRO, 0>::SMEM
+                            : PASS == EP_Z ? EnsZ<R, N>::SMEM
                             : PASS == EP_FWD_X ? XFBody<R, N, FM_HYDRO_STAGE, 0>::SMEM
                             : YBody<R, N, 1, 0>::SMEM;
+  static constexpr int WAVES = PASS == EP_Z ? EnsZ<R, N>::WAVES : 1;
 };
 
 // One launch covers every instance of grid size N (a size class): the
 // descriptors of the class are contiguous, `prefix` holds their block
 // offsets for this pass.
 template <typename R, int N, int PASS>
-__global__ __launch_bounds__((EnsBlock<R, N, PASS>::NT)) void kb_pass(const EnsInst<R>* __restrict__ inst,
+__global__ __launch_bounds__((EnsBlock<R, N, PASS>::NT))
+__attribute__((amdgpu_waves_per_eu(EnsBlock<R, N, PASS>::WAVES))) void kb_pass(const EnsInst<R>* __restrict__ inst,
                                                                      const int* __restrict__ prefix, int ninst,
                                                                      R dt, int stage) {
   __shared__ cpx<R> sm[EnsBlock<R, N, PASS>::SMEM];

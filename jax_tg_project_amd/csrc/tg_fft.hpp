@@ -13,6 +13,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 namespace tg {
 
 template <typename R>
@@ -352,6 +354,129 @@ __device__ __forceinline__ void fft_io(cpx<R>* buf, int LP, int nlines, const cp
   const int t = TFAST ? tid % TPL : tid / nlines;
   const bool active = TFAST ? (l < nlines) : (t < TPL);
   io_apply<R, N, E, DIR, SRC, DST_GLOBAL, PS>(buf, LP, l, t, active, tw, ld, st, typename PlanFor<N, E>::type{});
+}
+
+// ---------------------------------------------------------------- registers
+// Register-resident Stockham transform for lines owned by lanes of a single
+// wavefront.  The TPL = N / E lanes of a line keep its points in registers;
+// between stages they trade them through a line buffer in LDS that only
+// their own wavefront touches, so a wavefront-scope fence orders the
+// exchange (LDS instructions of one wavefront execute in issue order) and no
+// workgroup barrier is needed.
+//
+// Register order: before a stage of radix RAD, slot j*RAD + m of lane t
+// holds point t*(E/RAD) + j + m*(N/RAD).  After the last stage (radix RL)
+// slot j*RL + m holds output t*(E/RL) + j + m*(N/RL): exactly the input
+// order of a first stage of radix RL, so a plan and its reverse chain in
+// registers without an exchange.
+template <class Done, class Todo>
+struct RevImpl;
+template <int... Done>
+struct RevImpl<Radices<Done...>, Radices<>> {
+  using type = Radices<Done...>;
+};
+template <int... Done, int H, int... T>
+struct RevImpl<Radices<Done...>, Radices<H, T...>> {
+  using type = typename RevImpl<Radices<H, Done...>, Radices<T...>>::type;
+};
+template <class Plan>
+using ReversePlan = typename RevImpl<Radices<>, Plan>::type;
+
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Line buffer position of point n: one pad slot every 2^PS points keeps a
+// stage that writes E consecutive points per lane off a single bank.
+template <int PS>
+__device__ __forceinline__ int pidx(int n) { return n + (n >> PS); }
+
+// Per-stage twiddle rows in LDS: a stage (P, RAD) with P > 1 owns
+// (RAD - 1) * P entries, row m - 1 holding w^(m k N / (P RAD)) for k in
+// [0, P), so the lanes of a stage (consecutive k) read consecutive entries.
+// The rows of the stages follow each other in plan order.
+template <int N, int P, class Plan>
+struct StageTw;
+template <int N, int P>
+struct StageTw<N, P, Radices<>> {
+  static constexpr int SIZE = 0;
+  template <typename R>
+  __device__ __forceinline__ static void fill(cpx<R>*, const cpx<R>* __restrict__, int, int) {}
+};
+template <int N, int P, int R0, int... Rest>
+struct StageTw<N, P, Radices<R0, Rest...>> {
+  static constexpr int OWN = P > 1 ? (R0 - 1) * P : 0;
+  using Next = StageTw<N, P * R0, Radices<Rest...>>;
+  static constexpr int SIZE = OWN + Next::SIZE;
+  template <typename R>
+  __device__ __forceinline__ static void fill(cpx<R>* ts, const cpx<R>* __restrict__ tw, int tid, int nt) {
+    for (int i = tid; i < OWN; i += nt) {
+      const int m = i / P + 1, k = i % P;
+      ts[i] = tw[m * k * (N / (P * R0))];
+    }
+    Next::fill(ts + OWN, tw, tid, nt);
+  }
+};
+template <int N, class Plan>
+using PlanTw = StageTw<N, 1, Plan>;
+
+template <typename R, int N, int E, int DIR, int P, int RAD>
+__device__ __forceinline__ void rs_butterflies(cpx<R>* v, int t, const cpx<R>* ts) {
+  constexpr int NB = E / RAD;
+#pragma unroll
+  for (int j = 0; j < NB; ++j) {
+    if constexpr (P > 1) {
+      const int k = (t * NB + j) % P;
+#pragma unroll
+      for (int m = 1; m < RAD; ++m) {
+        cpx<R> w = ts[(m - 1) * P + k];
+        if constexpr (DIR > 0) w.y = -w.y;
+        v[j * RAD + m] = cmul(v[j * RAD + m], w);
+      }
+    }
+    dft<RAD, DIR>(&v[j * RAD]);
+  }
+}
+
+template <typename R, int N, int E, int P, int RAD, int PS>
+__device__ __forceinline__ void rs_put(const cpx<R>* v, int t, cpx<R>* lb) {
+  constexpr int NB = E / RAD;
+#pragma unroll
+  for (int j = 0; j < NB; ++j) {
+    const int b = t * NB + j, k = b % P, base = (b - k) * RAD + k;
+#pragma unroll
+    for (int m = 0; m < RAD; ++m) lb[pidx<PS>(base + m * P)] = v[j * RAD + m];
+  }
+}
+
+template <typename R, int N, int E, int RAD, int PS>
+__device__ __forceinline__ void rs_get(cpx<R>* v, int t, const cpx<R>* lb) {
+  constexpr int NB = E / RAD;
+#pragma unroll
+  for (int j = 0; j < NB; ++j)
+#pragma unroll
+    for (int m = 0; m < RAD; ++m) v[j * RAD + m] = lb[pidx<PS>(t * NB + j + m * (N / RAD))];
+}
+
+// ts: the twiddle rows of this plan (PlanTw layout).
+template <typename R, int N, int E, int DIR, int PS, int P, int R0, int... Rest>
+__device__ __forceinline__ void rs_stages(cpx<R>* v, int t, cpx<R>* lb, const cpx<R>* ts) {
+  rs_butterflies<R, N, E, DIR, P, R0>(v, t, ts);
+  if constexpr (sizeof...(Rest) > 0) {
+    constexpr int R1 = FirstRadix<Radices<Rest...>>::value;
+    rs_put<R, N, E, P, R0, PS>(v, t, lb);
+    wave_lds_sync();
+    rs_get<R, N, E, R1, PS>(v, t, lb);
+    wave_lds_sync();
+    rs_stages<R, N, E, DIR, PS, P * R0, Rest...>(v, t, lb, ts + (P > 1 ? (R0 - 1) * P : 0));
+  }
+}
+
+template <typename R, int N, int E, int DIR, int PS, int... Rs>
+__device__ __forceinline__ void reg_fft(cpx<R>* v, int t, cpx<R>* lb, const cpx<R>* ts, Radices<Rs...>) {
+  rs_stages<R, N, E, DIR, PS, 1, Rs...>(v, t, lb, ts);
 }
 
 // Transform `nlines` lines of length N held in LDS (buf[n*LP + l]).  The

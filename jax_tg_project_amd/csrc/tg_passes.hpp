@@ -603,6 +603,169 @@ __global__ __launch_bounds__((CfgZM<R, N, MODE, V>::NT)) void k_z(const cpx<R>* 
   B::run(sm, blockIdx.x, blockIdx.y, in, out, phys_in, phys_out, tw, lorentz);
 }
 
+// Hydro z pass with the lines in registers.  TPL = N / E lanes own one
+// (x, y) line and hold all three packed sequences u_p + i w_p of it (E
+// points each), so the product u x w is formed in registers right after the
+// inverse transforms, and the forward transforms start from there (their
+// plan is the inverse plan reversed, whose first-stage order is the inverse's
+// output order).  LDS serves only as a per-line exchange buffer between
+// stages, private to one wavefront: no workgroup barrier after the twiddle
+// table is staged.  Forward sequences per line: n_x + i n_y (split by
+// Hermitian symmetry on the way out) and n_z + 0 i.
+template <int N, int E>
+struct ZWPlan;
+template <> struct ZWPlan<256, 16> { using type = Radices<16, 16>; };
+template <> struct ZWPlan<256, 8> { using type = Radices<8, 8, 4>; };
+template <> struct ZWPlan<128, 16> { using type = Radices<16, 8>; };
+template <> struct ZWPlan<128, 8> { using type = Radices<8, 4, 4>; };
+template <> struct ZWPlan<64, 16> { using type = Radices<16, 4>; };
+template <> struct ZWPlan<64, 8> { using type = Radices<8, 8>; };
+template <> struct ZWPlan<32, 8> { using type = Radices<8, 4>; };
+template <> struct ZWPlan<96, 12> { using type = Radices<4, 4, 2, 3>; };
+template <> struct ZWPlan<48, 12> { using type = Radices<4, 4, 3>; };
+
+template <typename R, int N, int E_, int WPB>
+struct CfgZW {
+  static constexpr int E = E_;
+  static constexpr int TPL = N / E;          // lanes per line
+  static constexpr int LW = 64 / TPL;        // lines per wavefront
+  static_assert(N % E == 0 && 64 % TPL == 0, "a wavefront holds whole lines");
+  static constexpr int LINES = WPB * LW;     // lines per block
+  static constexpr int NT = 64 * WPB;
+  static constexpr int PS = ilog2(E);
+  static constexpr int USED = N + (N >> PS);
+  // line pitch: consecutive lines start 32 banks apart (8 mod 16 slots of
+  // 16 bytes for FP64, 16 mod 32 slots of 8 bytes for FP32)
+  static constexpr int LPL = sizeof(R) == 8 ? (USED + 7) / 16 * 16 + 8 : (USED + 15) / 32 * 32 + 16;
+  using IP = typename ZWPlan<N, E>::type;
+  using FP = ReversePlan<IP>;
+  static constexpr bool SAME = std::is_same<IP, FP>::value;
+  static constexpr int TWI = PlanTw<N, IP>::SIZE;
+  static constexpr int TWF = SAME ? 0 : PlanTw<N, FP>::SIZE;
+  static constexpr int SMEM = TWI + TWF + LINES * LPL;
+};
+
+// every full index k in [lo, hi] lies in the truncated band (|kz| > N/3)
+template <int N, int KZ>
+__device__ __forceinline__ constexpr bool band_zero(int lo, int hi) { return lo >= KZ && hi <= N - KZ; }
+
+template <typename R, int N, int E_, int WPB>
+struct ZWBody {
+  using G = Geo<N>;
+  using C = CfgZW<R, N, E_, WPB>;
+  using IP = typename C::IP;
+  using FP = typename C::FP;
+  static constexpr int SMEM = C::SMEM;
+  __device__ __forceinline__ static void run(cpx<R>* sm, int blk, const cpx<R>* __restrict__ in,
+                                             cpx<R>* __restrict__ out, const cpx<R>* __restrict__ tw) {
+    constexpr int E = C::E, TPL = C::TPL, LW = C::LW, PS = C::PS, KZ = G::KZ;
+    constexpr int KZP = kzp<R, N>();
+    constexpr int R1 = FirstRadix<IP>::value, NB1 = E / R1, S1 = N / R1;
+    const size_t NNKZ = size_t(N) * N * KZP;
+    cpx<R>* tsi = sm;
+    cpx<R>* tsf = C::SAME ? sm : sm + C::TWI;
+    PlanTw<N, IP>::fill(tsi, tw, threadIdx.x, C::NT);
+    if constexpr (!C::SAME) PlanTw<N, FP>::fill(tsf, tw, threadIdx.x, C::NT);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int t = lane % TPL, lw = lane / TPL;
+    cpx<R>* lb = sm + C::TWI + C::TWF + (wave * LW + lw) * C::LPL;
+    const size_t line = size_t(blk) * C::LINES + wave * LW + lw;
+    const cpx<R>* src = in + line * KZP;
+    cpx<R>* dst = out + line * KZP;
+
+    // ---- Hermitian-extended packed spectra: slot j*R1 + m holds point
+    // k = t*NB1 + j + m*S1 of sequence p = u_p + i w_p.
+    cpx<R> v[3][E];
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+#pragma unroll
+      for (int j = 0; j < NB1; ++j) {
+#pragma unroll
+        for (int m = 0; m < R1; ++m) {
+          cpx<R> z = {R(0), R(0)};
+          if (!band_zero<N, KZ>(j + m * S1, j + m * S1 + S1 - NB1)) {
+            const int k = t * NB1 + j + m * S1;
+            const bool neg = k > N / 2;
+            const int kzr = neg ? N - k : k;
+            const bool ok = kzr < KZ;
+            const int kz = ok ? kzr : KZ - 1;
+            cpx<R> a = src[p * NNKZ + kz], b = src[(p + 3) * NNKZ + kz];
+            if (k == 0) { a.y = R(0); b.y = R(0); }
+            if (neg) { a.y = -a.y; b.y = -b.y; }
+            if (ok) z = {a.x - b.y, a.y + b.x};
+          }
+          v[p][j * R1 + m] = z;
+        }
+      }
+    }
+    __syncthreads();  // twiddle rows staged
+#pragma unroll
+    for (int p = 0; p < 3; ++p) reg_fft<R, N, E, +1, PS>(v[p], t, lb, tsi, IP{});
+
+    // ---- n = u x w at the lane's E points, packed (n_x + i n_y), (n_z + 0 i)
+    cpx<R> a[E], c[E];
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      const R u[3] = {v[0][i].x, v[1][i].x, v[2][i].x};
+      const R w[3] = {v[0][i].y, v[1][i].y, v[2][i].y};
+      R n[3];
+      cross3(u, w, n);
+      a[i] = {n[0], n[1]};
+      c[i] = {n[2], R(0)};
+    }
+    reg_fft<R, N, E, -1, PS>(a, t, lb, tsf, FP{});
+    reg_fft<R, N, E, -1, PS>(c, t, lb, tsf, FP{});
+
+    // ---- n_z: real input, kept coefficients straight from registers
+#pragma unroll
+    for (int j = 0; j < NB1; ++j)
+#pragma unroll
+      for (int m = 0; m < R1; ++m) {
+        if (j + m * S1 < KZ) {
+          const int k = t * NB1 + j + m * S1;
+          if (k < KZ) dst[2 * NNKZ + k] = c[j * R1 + m];
+        }
+      }
+    // ---- n_x, n_y from the pair (k, N - k) of the packed transform
+#pragma unroll
+    for (int j = 0; j < NB1; ++j)
+#pragma unroll
+      for (int m = 0; m < R1; ++m) lb[pidx<PS>(t * NB1 + j + m * S1)] = a[j * R1 + m];
+    wave_lds_sync();
+#pragma unroll
+    for (int i = 0; i < (KZ + TPL - 1) / TPL; ++i) {
+      const int kz = t + i * TPL;
+      if (kz < KZ) {
+        const cpx<R> P = lb[pidx<PS>(kz)];
+        const cpx<R> Qc = lb[pidx<PS>(kz == 0 ? 0 : N - kz)];
+        dst[kz] = {R(0.5) * (P.x + Qc.x), R(0.5) * (P.y - Qc.y)};
+        dst[NNKZ + kz] = {R(0.5) * (P.y + Qc.y), R(0.5) * (Qc.x - P.x)};
+      }
+    }
+  }
+};
+
+// Default register z-pass shape of a grid size: points per lane and
+// wavefronts per block (E = 0: the grid uses the LDS-resident kernel).
+template <int N>
+struct ZwDefault {
+  static constexpr int E = N >= 128 ? 16 : (N == 48 || N == 96) ? 12 : (N >= 32 ? 8 : 0);
+  static constexpr int WPB = N >= 128 ? 4 : 1;
+};
+
+// Occupancy target: two wavefronts per SIMD with 16 points per lane (FP64
+// data alone fills 192 registers), three with 8.
+template <typename R, int E>
+constexpr int zw_waves_per_eu() { return E >= 12 ? (sizeof(R) == 8 ? 2 : 3) : (sizeof(R) == 8 ? 3 : 4); }
+
+template <typename R, int N, int E, int WPB>
+__global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(zw_waves_per_eu<R, E>())))
+void k_zw(const cpx<R>* __restrict__ in, cpx<R>* __restrict__ out, const cpx<R>* __restrict__ tw) {
+  using B = ZWBody<R, N, E, WPB>;
+  __shared__ cpx<R> sm[B::SMEM];
+  B::run(sm, blockIdx.x, in, out, tw);
+}
+
 // ----------------------------------------------------------------- pass 5
 // Forward transform along x of FF fields (3 or 6) for one (iy, kz tile),
 // keeping the K masked rows, followed by the spectral epilogue: Leray

@@ -154,15 +154,33 @@ struct Launch {
     TG_LAUNCHED();
     return TG_OK;
   }
+  // Register-resident hydro z pass: E points per lane, WPB wavefronts per block.
+  template <int E, int WPB>
+  static int zw(const cpx<R>* in, cpx<R>* out, const cpx<R>* tw, hipStream_t s) {
+    using C = CfgZW<R, N, E, WPB>;
+    static_assert((N * N) % C::LINES == 0, "lines per block must divide N^2");
+    hipLaunchKernelGGL((k_zw<R, N, E, WPB>), dim3(N * N / C::LINES), dim3(C::NT), 0, s, in, out, tw);
+    TG_LAUNCHED();
+    return TG_OK;
+  }
   // RHS modes take the tuned variant; edge modes (physical I/O, max |div|)
-  // always run variant 0.
+  // always run variant 0.  The hydro z pass is the register-resident kernel:
+  // N >= 128 variants 0 / 1 hold 16 / 8 points per lane (variant 2 is the
+  // LDS-resident kernel); smaller grids hold 8 (12 for N = 48, 96) points
+  // per lane, one wavefront per block.
   static int z(int mode, int var, const cpx<R>* in, cpx<R>* out, const R* pin, R* pout, const cpx<R>* tw, R lor,
                hipStream_t s) {
     switch (mode) {
       case ZM_HYDRO:
-        return with_variant<N>(var, [&](auto vc) {
-          return z_<ZM_HYDRO, decltype(vc)::value>(in, out, pin, pout, tw, lor, s);
-        });
+        if constexpr (N >= 128) {
+          if (var == 0) return zw<16, 4>(in, out, tw, s);
+          if (var == 1) return zw<8, 4>(in, out, tw, s);
+          return z_<ZM_HYDRO, 2>(in, out, pin, pout, tw, lor, s);
+        } else if constexpr (ZwDefault<N>::E > 0) {
+          return zw<ZwDefault<N>::E, ZwDefault<N>::WPB>(in, out, tw, s);
+        } else {
+          return z_<ZM_HYDRO, 0>(in, out, pin, pout, tw, lor, s);
+        }
       case ZM_MHD:
         return with_variant<N>(var, [&](auto vc) {
           return z_<ZM_MHD, decltype(vc)::value>(in, out, pin, pout, tw, lor, s);

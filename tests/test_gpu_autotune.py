@@ -56,3 +56,27 @@ def test_tuner_selects_and_caches(tmp_path, monkeypatch):
     assert again == best  # served from the cache
     small = Plan(32)
     assert autotune.tune(small)[0] == [0] * 5
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_z_variants_agree_256(dtype):
+    """The three hydro z-pass kernels at 256^3 (register-resident with 16 or
+    8 points per lane, LDS-resident) give the same RK4 step."""
+    from jax_tg_project_amd._lib import Plan
+    N = 256
+    p = Plan(N, dtype=dtype)
+    x = torch.arange(N, dtype=torch.float64, device="cuda") * (L / N)
+    X, Y, Z = torch.meshgrid(x, x, x, indexing="ij")
+    u = torch.stack([torch.sin(X) * torch.cos(Y) * torch.cos(Z) + 0.1 * torch.cos(7 * Y + 0.2) * torch.sin(40 * Z),
+                     -torch.cos(X) * torch.sin(Y) * torch.cos(Z) + 0.1 * torch.sin(30 * X - 0.4) * torch.cos(85 * Z),
+                     0.1 * torch.sin(2 * X + 1.0) * torch.cos(3 * Y) * torch.sin(63 * Z + 0.3)]).to(dtype).contiguous()
+    del X, Y, Z
+    outs = []
+    for v in range(3):
+        p.set_variants([0, 0, v, 0, 0])
+        uh = p.forward(u)
+        p.rk4(uh, None, nsteps=1, dt=2e-3, nu=1 / 400)
+        outs.append(p.inverse(uh).double().cpu().numpy())
+    tol = 1e-13 if dtype == torch.float64 else 2e-6
+    for x in outs[1:]:
+        assert rel_l2(x, outs[0]) < tol
