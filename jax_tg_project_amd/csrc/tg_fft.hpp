@@ -36,6 +36,33 @@ __device__ __forceinline__ cpx<R> cscale(cpx<R> a, R s) { return {a.x * s, a.y *
 template <typename R>
 __device__ __forceinline__ cpx<R> cmuli(cpx<R> a) { return {-a.y, a.x}; }
 
+// Global loads and stores with an optional nontemporal hint (NT), for
+// arrays that a pass streams through once and that are too large to stay in
+// the device's last-level cache until the next pass reads them (measured on
+// gfx950: a plain device copy moves 4.7 TB/s, 6.3 TB/s with both hints).
+template <typename R>
+using vec2_t = R __attribute__((ext_vector_type(2)));
+template <bool NT, typename R>
+__device__ __forceinline__ cpx<R> gld(const cpx<R>* p) {
+  if constexpr (NT) {
+    const vec2_t<R> v = __builtin_nontemporal_load(reinterpret_cast<const vec2_t<R>*>(p));
+    return {v.x, v.y};
+  } else {
+    return *p;
+  }
+}
+template <bool NT, typename R>
+__device__ __forceinline__ void gst(cpx<R>* p, cpx<R> c) {
+  if constexpr (NT) {
+    vec2_t<R> v;
+    v.x = c.x;
+    v.y = c.y;
+    __builtin_nontemporal_store(v, reinterpret_cast<vec2_t<R>*>(p));
+  } else {
+    *p = c;
+  }
+}
+
 // multiply by exp(DIR * 2 pi i / 4) = DIR * i
 template <int DIR, typename R>
 __device__ __forceinline__ cpx<R> rot4(cpx<R> a) {

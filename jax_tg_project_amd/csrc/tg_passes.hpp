@@ -38,6 +38,13 @@ constexpr int kzp() {
   return (Geo<N>::KZ + per_line - 1) / per_line * per_line;
 }
 
+// The intermediate arrays A and B of grids of 256^3 and up (0.2-0.6 GB
+// each in FP64) exceed the 256 MB last-level cache, so the passes stream
+// them with nontemporal loads and stores; the compact state (re-read by the
+// next pass while still cached) and smaller grids use ordinary accesses.
+template <int N>
+constexpr bool nt_mid() { return N >= 256; }
+
 // Launch shapes.  Grids of N >= 128 compile NVAR tile variants per pass;
 // the plan picks one per pass at run time (see the autotuner); smaller grids
 // compile variant 0 only.
@@ -231,7 +238,7 @@ struct XInvBody {
       };
       auto st = [&](int n, int ll, cpx<R> v) {
         const int izw = iz0 + ll;
-        if (izw < G::KZ) out[((size_t(f) * N + n) * G::K + iy) * kzp<R, N>() + izw] = v;
+        if (izw < G::KZ) gst<nt_mid<N>()>(&out[((size_t(f) * N + n) * G::K + iy) * kzp<R, N>() + izw], v);
       };
       fft_io<R, N, E, +1, 1, true>(buf, L, L, tws, ld, st);
     }
@@ -273,9 +280,9 @@ struct YBody {
       cpx<R> v;
       if constexpr (DIR > 0) {
         ok = ok && G::kept(n);
-        v = in[(plane * G::K + (ok ? G::compact_of(n) : 0)) * kzp<R, N>() + iz];
+        v = gld<nt_mid<N>()>(&in[(plane * G::K + (ok ? G::compact_of(n) : 0)) * kzp<R, N>() + iz]);
       } else {
-        v = in[(plane * N + n) * kzp<R, N>() + iz];
+        v = gld<nt_mid<N>()>(&in[(plane * N + n) * kzp<R, N>() + iz]);
       }
       return ok ? v : cpx<R>{R(0), R(0)};
     };
@@ -283,9 +290,9 @@ struct YBody {
       const int iz = iz0 + ll;
       if (iz >= G::KZ) return;
       if constexpr (DIR > 0) {
-        out[(plane * N + n) * kzp<R, N>() + iz] = v;
+        gst<nt_mid<N>()>(&out[(plane * N + n) * kzp<R, N>() + iz], v);
       } else {
-        if (G::kept(n)) out[(plane * G::K + G::compact_of(n)) * kzp<R, N>() + iz] = v;
+        if (G::kept(n)) gst<nt_mid<N>()>(&out[(plane * G::K + G::compact_of(n)) * kzp<R, N>() + iz], v);
       }
     };
     fft_io<R, N, E, DIR, 1, true>(buf, L, L, tws, ld, st);
@@ -689,7 +696,7 @@ struct ZWBody {
             const int kzr = neg ? N - k : k;
             const bool ok = kzr < KZ;
             const int kz = ok ? kzr : KZ - 1;
-            cpx<R> a = src[p * NNKZ + kz], b = src[(p + 3) * NNKZ + kz];
+            cpx<R> a = gld<nt_mid<N>()>(&src[p * NNKZ + kz]), b = gld<nt_mid<N>()>(&src[(p + 3) * NNKZ + kz]);
             if (k == 0) { a.y = R(0); b.y = R(0); }
             if (neg) { a.y = -a.y; b.y = -b.y; }
             if (ok) z = {a.x - b.y, a.y + b.x};
@@ -723,7 +730,7 @@ struct ZWBody {
       for (int m = 0; m < R1; ++m) {
         if (j + m * S1 < KZ) {
           const int k = t * NB1 + j + m * S1;
-          if (k < KZ) dst[2 * NNKZ + k] = c[j * R1 + m];
+          if (k < KZ) gst<nt_mid<N>()>(&dst[2 * NNKZ + k], c[j * R1 + m]);
         }
       }
     // ---- n_x, n_y from the pair (k, N - k) of the packed transform
@@ -738,8 +745,8 @@ struct ZWBody {
       if (kz < KZ) {
         const cpx<R> P = lb[pidx<PS>(kz)];
         const cpx<R> Qc = lb[pidx<PS>(kz == 0 ? 0 : N - kz)];
-        dst[kz] = {R(0.5) * (P.x + Qc.x), R(0.5) * (P.y - Qc.y)};
-        dst[NNKZ + kz] = {R(0.5) * (P.y + Qc.y), R(0.5) * (Qc.x - P.x)};
+        gst<nt_mid<N>()>(&dst[kz], cpx<R>{R(0.5) * (P.x + Qc.x), R(0.5) * (P.y - Qc.y)});
+        gst<nt_mid<N>()>(&dst[NNKZ + kz], cpx<R>{R(0.5) * (P.y + Qc.y), R(0.5) * (Qc.x - P.x)});
       }
     }
   }
@@ -852,7 +859,7 @@ struct XFBody {
   auto ld = [&](int n, int ln, int) -> cpx<R> {
     const int f = ln / L, izr = iz0 + ln % L;
     const bool ok = izr < G::KZ;
-    const cpx<R> v = in[((size_t(f) * N + n) * G::K + iy) * kzp<R, N>() + (ok ? izr : G::KZ - 1)];
+    const cpx<R> v = gld<nt_mid<N>()>(&in[((size_t(f) * N + n) * G::K + iy) * kzp<R, N>() + (ok ? izr : G::KZ - 1)]);
     return ok ? v : cpx<R>{R(0), R(0)};
   };
   fft_io<R, N, E, -1, 1, false>(buf, LP, LINES, tws, ld, NoIO{});
