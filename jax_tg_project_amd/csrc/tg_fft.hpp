@@ -409,6 +409,22 @@ struct RevImpl<Radices<Done...>, Radices<H, T...>> {
 template <class Plan>
 using ReversePlan = typename RevImpl<Radices<>, Plan>::type;
 
+// Value held by lane `src` of the same wavefront (every lane must execute
+// the call).
+__device__ __forceinline__ float lane_get(float v, int src) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_ds_bpermute(src << 2, __builtin_bit_cast(int, v)));
+}
+__device__ __forceinline__ double lane_get(double v, int src) {
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = unsigned(__builtin_amdgcn_ds_bpermute(src << 2, int(unsigned(b))));
+  const unsigned hi = unsigned(__builtin_amdgcn_ds_bpermute(src << 2, int(unsigned(b >> 32))));
+  return __builtin_bit_cast(double, (static_cast<unsigned long long>(hi) << 32) | lo);
+}
+template <typename R>
+__device__ __forceinline__ cpx<R> lane_get(cpx<R> v, int src) {
+  return {lane_get(v.x, src), lane_get(v.y, src)};
+}
+
 __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();

@@ -683,6 +683,44 @@ struct ZWBody {
     // ---- Hermitian-extended packed spectra: slot j*R1 + m holds point
     // k = t*NB1 + j + m*S1 of sequence p = u_p + i w_p.
     cpx<R> v[3][E];
+    if constexpr (NB1 == 1 && (TPL & (TPL - 1)) == 0) {
+      // Slots m < R1/2 hold k = t + m*TPL <= N/2 - 1, read from memory; slot
+      // m >= R1/2 holds N - k' of the lower slot R1-1-m of lane TPL - t (lane
+      // 0: its own slot R1 - m; k = N/2 is truncated), so every coefficient
+      // is read once and the mirror comes from the partner lane.
+      const int partner = (lane & ~(TPL - 1)) | ((TPL - t) & (TPL - 1));
+#pragma unroll
+      for (int p = 0; p < 3; ++p) {
+        cpx<R> w[R1 / 2];  // conj(a) + i conj(b): the value at N - k
+#pragma unroll
+        for (int m = 0; m < R1 / 2; ++m) {
+          cpx<R> z = {R(0), R(0)}, zm = {R(0), R(0)};
+          if (!band_zero<N, KZ>(m * S1, m * S1 + S1 - 1)) {
+            const int k = t + m * S1;
+            const bool ok = k < KZ;
+            const int kz = ok ? k : KZ - 1;
+            cpx<R> a = gld<nt_mid<N>()>(&src[p * NNKZ + kz]), b = gld<nt_mid<N>()>(&src[(p + 3) * NNKZ + kz]);
+            if (k == 0) { a.y = R(0); b.y = R(0); }
+            if (ok) {
+              z = {a.x - b.y, a.y + b.x};
+              zm = {a.x + b.y, b.x - a.y};
+            }
+          }
+          v[p][m] = z;
+          w[m] = zm;
+        }
+#pragma unroll
+        for (int m = R1 / 2; m < R1; ++m) {
+          cpx<R> z = {R(0), R(0)};
+          if (!band_zero<N, KZ>(m * S1, m * S1 + S1 - 1)) {
+            const cpx<R> f = lane_get(w[R1 - 1 - m], partner);
+            const cpx<R> own = (m == R1 / 2) ? cpx<R>{R(0), R(0)} : w[(R1 - m) % R1];
+            z = t == 0 ? own : f;
+          }
+          v[p][m] = z;
+        }
+      }
+    } else {
 #pragma unroll
     for (int p = 0; p < 3; ++p) {
 #pragma unroll
@@ -704,6 +742,7 @@ struct ZWBody {
           v[p][j * R1 + m] = z;
         }
       }
+    }
     }
     __syncthreads();  // twiddle rows staged
 #pragma unroll
@@ -760,10 +799,14 @@ struct ZwDefault {
   static constexpr int WPB = N >= 128 ? 4 : 1;
 };
 
-// Occupancy target: two wavefronts per SIMD with 16 points per lane (FP64
-// data alone fills 192 registers), three with 8.
+// Occupancy target (wavefronts per SIMD, i.e. a register budget of 512 /
+// target per lane): FP64 with 16 points per lane keeps 192 registers of
+// line data alone; the 12-point plans (radix 3, no mirrored loads) need the
+// larger budget in FP32 too.
 template <typename R, int E>
-constexpr int zw_waves_per_eu() { return E >= 12 ? (sizeof(R) == 8 ? 2 : 3) : (sizeof(R) == 8 ? 3 : 4); }
+constexpr int zw_waves_per_eu() {
+  return sizeof(R) == 8 ? (E >= 12 ? 2 : 3) : (E == 12 ? 3 : 4);
+}
 
 template <typename R, int N, int E, int WPB>
 __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(zw_waves_per_eu<R, E>())))
