@@ -25,8 +25,8 @@ if ROOT not in sys.path:
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hydro_256_fp64_traffic.txt")
 KERNEL_OF_PASS = {"inv_x": "k_inv_x<double, 256, 0, {v}>", "inv_y": "k_y<double, 256, 1, {v}>",
-                  "z": "k_z<double, 256, 0, {v}>", "fwd_y": "k_y<double, 256, -1, {v}>",
-                  "fwd_x": "k_fwd_x<double, 256, 0, {v}>"}
+                  "z": ["k_zw<double, 256, 16, 4>", "k_zw<double, 256, 8, 4>", "k_z<double, 256, 0, 2>"],
+                  "fwd_y": "k_y<double, 256, -1, {v}>", "fwd_x": "k_fwd_x<double, 256, 0, {v}>"}
 PASS_INDEX = {"inv_x": 0, "inv_y": 1, "z": 2, "fwd_y": 3, "fwd_x": 4}
 
 
@@ -38,7 +38,9 @@ def measured_traffic(pass_name, variants):
             rows = [ln.rstrip("\n").split("\t") for ln in f if not ln.startswith("#")]
     except OSError:
         return None
-    want = "tg::" + KERNEL_OF_PASS[pass_name].format(v=variants[PASS_INDEX[pass_name]])
+    v = variants[PASS_INDEX[pass_name]]
+    k = KERNEL_OF_PASS[pass_name]
+    want = "tg::" + (k[v] if isinstance(k, list) else k.format(v=v))
     for r in rows:
         if len(r) == 3 and r[0] == want:
             return float(r[1]) + float(r[2])

@@ -168,7 +168,8 @@ int tg_batch_rk4_steps(tg_batch* batch, int nsteps, double dt, int use_graph, vo
  * tg_vjp: out = J_rhs(x)^T a = mask F(w x q) + i k x mask F(q x u) - nu k^2 a,
  *   q = F^-1(P a), u = F^-1 x, w = curl u (hydro plans). */
 int tg_vjp(tg_plan* plan, const void* x_hat, const void* a_hat, double nu, void* out_hat, void* ws, void* stream);
-/* Bytes of adjoint workspace for tg_adjoint_grad(nsteps, seg). */
+/* Bytes of the adjoint workspace adj_ws that tg_adjoint_forward and
+ * tg_adjoint_backward need for (nsteps, seg). */
 int tg_adjoint_workspace(tg_plan* plan, int nsteps, int seg, long long* bytes);
 /* Forward sweep: J = E(T) (objective 0) or Z(T) (1) after nsteps RK4 steps of
  * size dt from x0_hat with viscosity nu, keeping a checkpoint every `seg`
