@@ -311,7 +311,7 @@ __device__ __forceinline__ void io_stage(cpx<R>* buf, int LP, int l, int t, bool
       const int base = (b - k) * RAD + k;
 #pragma unroll
       for (int m = 0; m < RAD; ++m) {
-        if constexpr (DST_GLOBAL) st(base + m * P, l, v[j][m]);
+        if constexpr (DST_GLOBAL) st(base + m * P, l, v[j][m], j * RAD + m);  // slot: compile-time
         else buf[lidx<PS>(base + m * P, l, LP)] = v[j][m];
       }
     }
@@ -342,6 +342,14 @@ template <int R0, int... Rest>
 struct FirstRadix<Radices<R0, Rest...>> {
   static constexpr int value = R0;
 };
+template <class Plan>
+struct LastRadix;
+template <int R0>
+struct LastRadix<Radices<R0>> {
+  static constexpr int value = R0;
+};
+template <int R0, int R1, int... Rest>
+struct LastRadix<Radices<R0, R1, Rest...>> : LastRadix<Radices<R1, Rest...>> {};
 __host__ __device__ constexpr int ilog2(int v) { return v <= 1 ? 0 : 1 + ilog2(v / 2); }
 // Pad shift for a slice-fastest transform of length N with E points per thread.
 template <int N, int E>
@@ -350,8 +358,8 @@ constexpr int pad_shift() { return ilog2(FirstRadix<typename PlanFor<N, E>::type
 // Transform `nlines` lines whose first stage reads through `ld(n, l, slot)`
 // (slot = the thread's register slot of that point, 0..E-1, fixed for a
 // given thread and n) and
-// (if DST_GLOBAL) whose last stage writes through `st(n, l, v)` in natural
-// order; intermediate stages exchange through LDS (buf[n*LP + l]).  The
+// (if DST_GLOBAL) whose last stage writes through `st(n, l, v, slot)` in
+// natural order; intermediate stages exchange through LDS (buf[n*LP + l]).  The
 // caller guarantees the LDS buffer is free on entry (barrier after any
 // earlier use).  With DST_GLOBAL the call ends without a barrier.
 struct NoIO {

@@ -236,7 +236,7 @@ struct XInvBody {
         v = cscale(v, scale);
         return ok ? v : cpx<R>{R(0), R(0)};
       };
-      auto st = [&](int n, int ll, cpx<R> v) {
+      auto st = [&](int n, int ll, cpx<R> v, int) {
         const int izw = iz0 + ll;
         if (izw < G::KZ) gst<nt_mid<N>()>(&out[((size_t(f) * N + n) * G::K + iy) * kzp<R, N>() + izw], v);
       };
@@ -286,7 +286,7 @@ struct YBody {
       }
       return ok ? v : cpx<R>{R(0), R(0)};
     };
-    auto st = [&](int n, int ll, cpx<R> v) {
+    auto st = [&](int n, int ll, cpx<R> v, int) {
       const int iz = iz0 + ll;
       if (iz >= G::KZ) return;
       if constexpr (DIR > 0) {
@@ -1014,6 +1014,88 @@ __global__ __launch_bounds__((CfgXF<R, N, V>::NT)) void k_fwd_x(const cpx<R>* __
                                                               StageArgs<R> a,
                                                               const cpx<R>* __restrict__ tw) {
   using B = XFBody<R, N, MODE, V>;
+  __shared__ cpx<R> sm[B::SMEM];
+  B::run(sm, blockIdx.x, blockIdx.y, in, a, tw);
+}
+
+// Forward x + hydro stage epilogue with the transformed lines kept in
+// registers: the block transforms the three fields of its L kz columns one
+// after the other through one N x L LDS tile; the last stage of each leaves
+// the thread's E outputs of its column in registers, so the projection,
+// viscous term and RK4 update run on the thread's own (kx, kz) points with
+// no LDS round trip.  LDS holds one field instead of three, so several
+// blocks share a CU and overlap their load, transform and update phases.
+template <typename R, int N>
+struct CfgXR {
+  static constexpr int E = 8;
+  static constexpr int L = sizeof(R) == 8 ? 8 : 16;  // one 128-byte line of kz per row
+  static constexpr int NT = L * N / E;
+};
+
+template <typename R, int N>
+struct XRBody {
+  using G = Geo<N>;
+  using C = CfgXR<R, N>;
+  static constexpr int SMEM = N * C::L + N;
+  __device__ __forceinline__ static void run(cpx<R>* sm, int bx, int by, const cpx<R>* __restrict__ in,
+                                             const StageArgs<R>& a, const cpx<R>* __restrict__ tw) {
+    constexpr int L = C::L, E = C::E;
+    using Pl = typename PlanFor<N, E>::type;
+    constexpr int RL = LastRadix<Pl>::value, NBL = E / RL, SL = N / RL;
+    cpx<R>* buf = sm;
+    cpx<R>* tws = sm + N * L;
+    const int iz0 = bx * L, iy = by;
+    const int tid = threadIdx.x, l = tid % L, t = tid / L;
+    const int izr = iz0 + l;
+    const bool colok = izr < G::KZ;
+    const int iz = colok ? izr : G::KZ - 1;
+    stage_twiddles<R, N>(tws, tw);
+    cpx<R> o[3][E];
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+      if (f) __syncthreads();  // the previous field's last LDS reads are done
+      auto ld = [&](int n, int, int) -> cpx<R> {
+        const cpx<R> v = gld<nt_mid<N>()>(&in[((size_t(f) * N + n) * G::K + iy) * kzp<R, N>() + iz]);
+        return colok ? v : cpx<R>{R(0), R(0)};
+      };
+      auto st = [&](int, int, cpx<R> v, int slot) { o[f][slot] = v; };
+      fft_io<R, N, E, -1, 1, true>(buf, L, L, tws, ld, st);
+    }
+    if (!colok) return;
+    const int cpz = a.cpz ? a.cpz : G::KZ;
+    const size_t cs = size_t(G::K) * G::K * cpz;
+    const R ky = a.ks * R(G::kint(iy)), kz = a.ks * R(iz);
+    // slot j*RL + m holds output n = t*NBL + j + m*SL
+#pragma unroll
+    for (int j = 0; j < NBL; ++j) {
+#pragma unroll
+      for (int m = 0; m < RL; ++m) {
+        const int n = t * NBL + j + m * SL;
+        if (!G::kept(n)) continue;
+        const int ix = G::compact_of(n);
+        const size_t pt = (size_t(ix) * G::K + iy) * cpz + iz;
+        cpx<R> v[3] = {o[0][j * RL + m], o[1][j * RL + m], o[2][j * RL + m]};
+        const R kx = a.ks * R(G::kint(ix));
+        const R k2 = kx * kx + ky * ky + kz * kz;
+        const R k2s = (k2 == R(0)) ? R(1) : k2;
+        project(kx, ky, kz, k2s, v);
+        cpx<R> ru[3];
+        const R nk = a.nu * k2;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const cpx<R> s = a.us[c * cs + pt];
+          ru[c] = {v[c].x - nk * s.x, v[c].y - nk * s.y};
+        }
+        rk4_update(a.stage, a.dt, pt, cs, ru, a.u0, a.unext, a.uacc);
+      }
+    }
+  }
+};
+
+template <typename R, int N>
+__global__ __launch_bounds__((CfgXR<R, N>::NT)) void k_fwd_xr(const cpx<R>* __restrict__ in, StageArgs<R> a,
+                                                             const cpx<R>* __restrict__ tw) {
+  using B = XRBody<R, N>;
   __shared__ cpx<R> sm[B::SMEM];
   B::run(sm, blockIdx.x, blockIdx.y, in, a, tw);
 }

@@ -193,7 +193,16 @@ struct Launch {
     }
   }
   static int nmax() { return (N / CfgZM<R, N, ZM_MAXABS, 0>::LZ) * N; }
+  // Hydro RK4 stage variant 1 of N >= 128 is the register-epilogue kernel.
   static int fwd_x(int mode, int var, const cpx<R>* in, const StageArgs<R>& a, const cpx<R>* tw, hipStream_t s) {
+    if constexpr (N >= 128) {
+      if (mode == FM_HYDRO_STAGE && var == 1) {
+        using C = CfgXR<R, N>;
+        hipLaunchKernelGGL((k_fwd_xr<R, N>), dim3(cdiv(G::KZ, C::L), G::K), dim3(C::NT), 0, s, in, a, tw);
+        TG_LAUNCHED();
+        return TG_OK;
+      }
+    }
     return with_variant<N>(var, [&](auto vc) {
       constexpr int V = decltype(vc)::value;
       using C = CfgXF<R, N, V>;

@@ -59,9 +59,11 @@ def test_tuner_selects_and_caches(tmp_path, monkeypatch):
 
 
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
-def test_z_variants_agree_256(dtype):
+def test_z_and_fwd_x_variants_agree_256(dtype):
     """The three hydro z-pass kernels at 256^3 (register-resident with 16 or
-    8 points per lane, LDS-resident) give the same RK4 step."""
+    8 points per lane, LDS-resident) and the three forward-x stage kernels
+    (one of them with the epilogue on register-held lines) give the same RK4
+    step."""
     from jax_tg_project_amd._lib import Plan
     N = 256
     p = Plan(N, dtype=dtype)
@@ -72,8 +74,8 @@ def test_z_variants_agree_256(dtype):
                      0.1 * torch.sin(2 * X + 1.0) * torch.cos(3 * Y) * torch.sin(63 * Z + 0.3)]).to(dtype).contiguous()
     del X, Y, Z
     outs = []
-    for v in range(3):
-        p.set_variants([0, 0, v, 0, 0])
+    for v in [[0, 0, 0, 0, 0], [0, 0, 1, 0, 1], [0, 0, 2, 0, 2], [0, 0, 0, 0, 1]]:
+        p.set_variants(v)
         uh = p.forward(u)
         p.rk4(uh, None, nsteps=1, dt=2e-3, nu=1 / 400)
         outs.append(p.inverse(uh).double().cpu().numpy())
