@@ -56,11 +56,12 @@ template <int N>
 constexpr int base_e() { return (N == 48 || N == 96) ? 12 : (N >= 128 ? 16 : (N == 4 ? 4 : 8)); }
 
 // X/Y passes: L columns of kz per block, E points per thread.
-//   V0: 16 columns x 16 points   V1: 8 columns x 8 points   V2: 32 columns x 16 points
+//   V0: 16 columns x 16 points   V1: one 128-byte line of columns (8 FP64,
+//   16 FP32) x 8 points   V2: 32 columns x 16 points
 template <typename R, int N, int V>
 struct CfgXY {
   static constexpr int E = (V == 1) ? 8 : base_e<N>();
-  static constexpr int L = V == 1 ? 8 : (V == 2 ? 32 : 16);
+  static constexpr int L = V == 1 ? (sizeof(R) == 4 ? 16 : 8) : (V == 2 ? 32 : 16);
   static constexpr int NT = L * N / E;
 };
 // Z pass: NB = 3 x (hydro lines per block); E points per thread for the
@@ -245,8 +246,14 @@ struct XInvBody {
   }
 };
 
+// FP32 variant 1 (8 points per thread) fits four wavefronts per SIMD (not
+// the adjoint mode, whose nine fields need more registers).
+template <typename R, int MODE, int V>
+constexpr int inv_x_waves_per_eu() { return (V == 1 && sizeof(R) == 4 && MODE != XM_ADJ) ? 4 : 1; }
+
 template <typename R, int N, int MODE, int V>
-__global__ __launch_bounds__((CfgXY<R, N, V>::NT)) void k_inv_x(const cpx<R>* __restrict__ su,
+__global__ __launch_bounds__((CfgXY<R, N, V>::NT)) __attribute__((amdgpu_waves_per_eu(inv_x_waves_per_eu<R, MODE, V>())))
+void k_inv_x(const cpx<R>* __restrict__ su,
                                                               const cpx<R>* __restrict__ sb,
                                                               cpx<R>* __restrict__ out,
                                                               const cpx<R>* __restrict__ tw, R ks,
