@@ -120,6 +120,37 @@ def cpu_baseline(N, dt, nu, gpu_one_step=None):
     return out, parity
 
 
+def run_mhd(N, steps, dt, nu, eta, dtype, lorentz_sign=1.0):
+    """MHD TGV: u = TG, b = (cos x sin y sin z, sin x cos y sin z,
+    -2 sin x sin y cos z) (= -curl u_TG), nu = eta, `steps` RK4 steps on the
+    compact spectra after one untimed warm-up step; FP64 diagnostics."""
+    import torch
+    from jax_tg_project_amd._lib import Plan
+    from jax_tg_project_amd import autotune
+    plan = Plan(N, dtype=dtype, mhd=True)
+    tuned = autotune.tune(plan, use_cache=False)[0]
+    x = torch.arange(N, dtype=torch.float64, device=plan.device) * (2.0 * math.pi / N)
+    X, Y, Z = torch.meshgrid(x, x, x, indexing="ij")
+    u = torch.stack([torch.sin(X) * torch.cos(Y) * torch.cos(Z), -torch.cos(X) * torch.sin(Y) * torch.cos(Z),
+                     torch.zeros_like(X)]).to(dtype).contiguous()
+    b = torch.stack([torch.cos(X) * torch.sin(Y) * torch.sin(Z), torch.sin(X) * torch.cos(Y) * torch.sin(Z),
+                     -2.0 * torch.sin(X) * torch.sin(Y) * torch.cos(Z)]).to(dtype).contiguous()
+    del X, Y, Z
+    uh, bh = plan.forward(u), plan.forward(b)
+    w, v = uh.clone(), bh.clone()
+    plan.rk4(w, v, nsteps=1, dt=dt, nu=nu, eta=eta, lorentz_sign=lorentz_sign)
+    del w, v
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plan.rk4(uh, bh, nsteps=steps, dt=dt, nu=nu, eta=eta, lorentz_sign=lorentz_sign)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    d = plan.diagnostics(uh, bh)
+    return {"steps_per_s": steps / wall, "wall_s": wall, "tile_variants": tuned,
+            "kinetic_energy": d["kinetic_energy"], "magnetic_energy": d["magnetic_energy"],
+            "cross_helicity": d["cross_helicity"], "div_u_max": d["div_u_max"], "div_b_max": d["div_b_max"]}
+
+
 def run_ensemble(ninst, steps, dt, seed=0):
     """Seeded 64-instance FP32 ensemble (sizes {32,48,64,96}, nu log-uniform
     in [1/1600, 1/100], TG + |k|<=4 perturbation of rms 0.05): wall time of
@@ -194,6 +225,7 @@ def main():
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-mhd", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="only run a 1 GiB device copy (counter calibration) and 3 tuned FP64 steps, "
@@ -284,6 +316,19 @@ def main():
         res["ensemble"] = run_ensemble(64, 200, 5e-3)
         torch.cuda.empty_cache()
 
+    if not args.no_mhd:
+        # MHD TGV at 128^3 (nu = eta = 1/200, dt = 2e-3, 300 steps) with the
+        # physical Lorentz sign; FP32 errors stated against the FP64 run.
+        m64 = run_mhd(128, 300, 2e-3, 1.0 / 200, 1.0 / 200, torch.float64)
+        m32 = run_mhd(128, 300, 2e-3, 1.0 / 200, 1.0 / 200, torch.float32)
+        for q in ("kinetic_energy", "magnetic_energy"):
+            m32[q + "_rel_err_vs_fp64"] = abs(m32[q] - m64[q]) / max(abs(m64[q]), 1e-300)
+        # H_c vanishes for this symmetric state: error relative to its bound 2 sqrt(E_k E_m)
+        m32["cross_helicity_err_vs_fp64_over_bound"] = abs(m32["cross_helicity"] - m64["cross_helicity"]) / (
+            2.0 * math.sqrt(m64["kinetic_energy"] * m64["magnetic_energy"]))
+        res["mhd_128"] = {"config": "MHD TGV N=128^3, b = -curl u_TG, nu = eta = 1/200, dt = 2e-3, 300 RK4 steps, "
+                                    "lorentz_sign = +1 (j x b)", "fp64": m64, "fp32": m32,
+                          "fp32_over_fp64": m32["steps_per_s"] / m64["steps_per_s"]}
     if not args.no_adjoint:
         res["adjoint"] = [run_adjoint(128, 200, 5e-3, 1.0 / 400, dt_) for dt_ in (torch.float64, torch.float32)]
         torch.cuda.empty_cache()
