@@ -663,6 +663,104 @@ struct CfgZW {
 template <int N, int KZ>
 __device__ __forceinline__ constexpr bool band_zero(int lo, int hi) { return lo >= KZ && hi <= N - KZ; }
 
+// Load the Hermitian-extended packed sequence a + i b of fields FA, FB (FB <
+// 0: b = 0) of one line into registers in the first-stage order of plan IP:
+// slot j*R1 + m holds point k = t*NB1 + j + m*S1.
+template <typename R, int N, class C, int FA, int FB>
+__device__ __forceinline__ void zw_load(cpx<R>* v, const cpx<R>* __restrict__ src, size_t NNKZ, int t, int lane) {
+  using IP = typename C::IP;
+  constexpr int E = C::E, TPL = C::TPL, KZ = Geo<N>::KZ;
+  constexpr int R1 = FirstRadix<IP>::value, NB1 = E / R1, S1 = N / R1;
+  auto fetch = [&](int kz, cpx<R>& a, cpx<R>& b) {
+    a = gld<nt_mid<N>()>(&src[FA * NNKZ + kz]);
+    if constexpr (FB >= 0) b = gld<nt_mid<N>()>(&src[FB * NNKZ + kz]);
+    else b = {R(0), R(0)};
+  };
+  if constexpr (NB1 == 1 && (TPL & (TPL - 1)) == 0) {
+    // Slots m < R1/2 hold k = t + m*TPL <= N/2 - 1, read from memory; slot
+    // m >= R1/2 holds N - k' of the lower slot R1-1-m of lane TPL - t (lane
+    // 0: its own slot R1 - m; k = N/2 is truncated), so every coefficient
+    // is read once and the mirror comes from the partner lane.
+    const int partner = (lane & ~(TPL - 1)) | ((TPL - t) & (TPL - 1));
+    cpx<R> w[R1 / 2];  // conj(a) + i conj(b): the value at N - k
+#pragma unroll
+    for (int m = 0; m < R1 / 2; ++m) {
+      cpx<R> z = {R(0), R(0)}, zm = {R(0), R(0)};
+      if (!band_zero<N, KZ>(m * S1, m * S1 + S1 - 1)) {
+        const int k = t + m * S1;
+        const bool ok = k < KZ;
+        cpx<R> a, b;
+        fetch(ok ? k : KZ - 1, a, b);
+        if (k == 0) { a.y = R(0); b.y = R(0); }
+        if (ok) {
+          z = {a.x - b.y, a.y + b.x};
+          zm = {a.x + b.y, b.x - a.y};
+        }
+      }
+      v[m] = z;
+      w[m] = zm;
+    }
+#pragma unroll
+    for (int m = R1 / 2; m < R1; ++m) {
+      cpx<R> z = {R(0), R(0)};
+      if (!band_zero<N, KZ>(m * S1, m * S1 + S1 - 1)) {
+        const cpx<R> f = lane_get(w[R1 - 1 - m], partner);
+        const cpx<R> own = (m == R1 / 2) ? cpx<R>{R(0), R(0)} : w[(R1 - m) % R1];
+        z = t == 0 ? own : f;
+      }
+      v[m] = z;
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < NB1; ++j) {
+#pragma unroll
+      for (int m = 0; m < R1; ++m) {
+        cpx<R> z = {R(0), R(0)};
+        if (!band_zero<N, KZ>(j + m * S1, j + m * S1 + S1 - NB1)) {
+          const int k = t * NB1 + j + m * S1;
+          const bool neg = k > N / 2;
+          const int kzr = neg ? N - k : k;
+          const bool ok = kzr < KZ;
+          cpx<R> a, b;
+          fetch(ok ? kzr : KZ - 1, a, b);
+          if (k == 0) { a.y = R(0); b.y = R(0); }
+          if (neg) { a.y = -a.y; b.y = -b.y; }
+          if (ok) z = {a.x - b.y, a.y + b.x};
+        }
+        v[j * R1 + m] = z;
+      }
+    }
+  }
+}
+
+// Store the kept kz of the two real fields packed in the forward transform
+// x (output order of plan FP) into fields FA (real part) and FB: they are
+// split by Hermitian symmetry from the pair (k, N - k), through the line
+// buffer.  Ends with the buffer free again.
+template <typename R, int N, class C, int FA, int FB>
+__device__ __forceinline__ void zw_store_pair(const cpx<R>* x, cpx<R>* lb, cpx<R>* __restrict__ dst, size_t NNKZ,
+                                              int t) {
+  using IP = typename C::IP;
+  constexpr int E = C::E, TPL = C::TPL, PS = C::PS, KZ = Geo<N>::KZ;
+  constexpr int R1 = FirstRadix<IP>::value, NB1 = E / R1, S1 = N / R1;
+#pragma unroll
+  for (int j = 0; j < NB1; ++j)
+#pragma unroll
+    for (int m = 0; m < R1; ++m) lb[pidx<PS>(t * NB1 + j + m * S1)] = x[j * R1 + m];
+  wave_lds_sync();
+#pragma unroll
+  for (int i = 0; i < (KZ + TPL - 1) / TPL; ++i) {
+    const int kz = t + i * TPL;
+    if (kz < KZ) {
+      const cpx<R> P = lb[pidx<PS>(kz)];
+      const cpx<R> Qc = lb[pidx<PS>(kz == 0 ? 0 : N - kz)];
+      gst<nt_mid<N>()>(&dst[FA * NNKZ + kz], cpx<R>{R(0.5) * (P.x + Qc.x), R(0.5) * (P.y - Qc.y)});
+      gst<nt_mid<N>()>(&dst[FB * NNKZ + kz], cpx<R>{R(0.5) * (P.y + Qc.y), R(0.5) * (Qc.x - P.x)});
+    }
+  }
+  wave_lds_sync();
+}
+
 template <typename R, int N, int E_, int WPB>
 struct ZWBody {
   using G = Geo<N>;
@@ -687,70 +785,11 @@ struct ZWBody {
     const cpx<R>* src = in + line * KZP;
     cpx<R>* dst = out + line * KZP;
 
-    // ---- Hermitian-extended packed spectra: slot j*R1 + m holds point
-    // k = t*NB1 + j + m*S1 of sequence p = u_p + i w_p.
+    // ---- the packed sequences u_p + i w_p
     cpx<R> v[3][E];
-    if constexpr (NB1 == 1 && (TPL & (TPL - 1)) == 0) {
-      // Slots m < R1/2 hold k = t + m*TPL <= N/2 - 1, read from memory; slot
-      // m >= R1/2 holds N - k' of the lower slot R1-1-m of lane TPL - t (lane
-      // 0: its own slot R1 - m; k = N/2 is truncated), so every coefficient
-      // is read once and the mirror comes from the partner lane.
-      const int partner = (lane & ~(TPL - 1)) | ((TPL - t) & (TPL - 1));
-#pragma unroll
-      for (int p = 0; p < 3; ++p) {
-        cpx<R> w[R1 / 2];  // conj(a) + i conj(b): the value at N - k
-#pragma unroll
-        for (int m = 0; m < R1 / 2; ++m) {
-          cpx<R> z = {R(0), R(0)}, zm = {R(0), R(0)};
-          if (!band_zero<N, KZ>(m * S1, m * S1 + S1 - 1)) {
-            const int k = t + m * S1;
-            const bool ok = k < KZ;
-            const int kz = ok ? k : KZ - 1;
-            cpx<R> a = gld<nt_mid<N>()>(&src[p * NNKZ + kz]), b = gld<nt_mid<N>()>(&src[(p + 3) * NNKZ + kz]);
-            if (k == 0) { a.y = R(0); b.y = R(0); }
-            if (ok) {
-              z = {a.x - b.y, a.y + b.x};
-              zm = {a.x + b.y, b.x - a.y};
-            }
-          }
-          v[p][m] = z;
-          w[m] = zm;
-        }
-#pragma unroll
-        for (int m = R1 / 2; m < R1; ++m) {
-          cpx<R> z = {R(0), R(0)};
-          if (!band_zero<N, KZ>(m * S1, m * S1 + S1 - 1)) {
-            const cpx<R> f = lane_get(w[R1 - 1 - m], partner);
-            const cpx<R> own = (m == R1 / 2) ? cpx<R>{R(0), R(0)} : w[(R1 - m) % R1];
-            z = t == 0 ? own : f;
-          }
-          v[p][m] = z;
-        }
-      }
-    } else {
-#pragma unroll
-    for (int p = 0; p < 3; ++p) {
-#pragma unroll
-      for (int j = 0; j < NB1; ++j) {
-#pragma unroll
-        for (int m = 0; m < R1; ++m) {
-          cpx<R> z = {R(0), R(0)};
-          if (!band_zero<N, KZ>(j + m * S1, j + m * S1 + S1 - NB1)) {
-            const int k = t * NB1 + j + m * S1;
-            const bool neg = k > N / 2;
-            const int kzr = neg ? N - k : k;
-            const bool ok = kzr < KZ;
-            const int kz = ok ? kzr : KZ - 1;
-            cpx<R> a = gld<nt_mid<N>()>(&src[p * NNKZ + kz]), b = gld<nt_mid<N>()>(&src[(p + 3) * NNKZ + kz]);
-            if (k == 0) { a.y = R(0); b.y = R(0); }
-            if (neg) { a.y = -a.y; b.y = -b.y; }
-            if (ok) z = {a.x - b.y, a.y + b.x};
-          }
-          v[p][j * R1 + m] = z;
-        }
-      }
-    }
-    }
+    zw_load<R, N, C, 0, 3>(v[0], src, NNKZ, t, lane);
+    zw_load<R, N, C, 1, 4>(v[1], src, NNKZ, t, lane);
+    zw_load<R, N, C, 2, 5>(v[2], src, NNKZ, t, lane);
     __syncthreads();  // twiddle rows staged
 #pragma unroll
     for (int p = 0; p < 3; ++p) reg_fft<R, N, E, +1, PS>(v[p], t, lb, tsi, IP{});
@@ -780,21 +819,65 @@ struct ZWBody {
         }
       }
     // ---- n_x, n_y from the pair (k, N - k) of the packed transform
+    zw_store_pair<R, N, C, 0, 1>(a, lb, dst, NNKZ, t);
+  }
+};
+
+// Adjoint z pass with the lines in registers: per line the nine inverse
+// fields u (0-2), w = curl u (3-5) and q (6-8) as five packed sequences
+// (u_x + i u_y), (u_z + i w_x), (w_y + i w_z), (q_x + i q_y), (q_z + 0 i);
+// products p1 = w x q and p2 = q x u; three forward sequences (p1_x + i
+// p1_y), (p1_z + i p2_x), (p2_y + i p2_z) stored as fields 0-5.
+template <typename R, int N, int E_, int WPB>
+struct ZWAdjBody {
+  using G = Geo<N>;
+  using C = CfgZW<R, N, E_, WPB>;
+  using IP = typename C::IP;
+  using FP = typename C::FP;
+  static constexpr int SMEM = C::SMEM;
+  __device__ __forceinline__ static void run(cpx<R>* sm, int blk, const cpx<R>* __restrict__ in,
+                                             cpx<R>* __restrict__ out, const cpx<R>* __restrict__ tw) {
+    constexpr int E = C::E, TPL = C::TPL, LW = C::LW, PS = C::PS;
+    constexpr int KZP = kzp<R, N>();
+    const size_t NNKZ = size_t(N) * N * KZP;
+    cpx<R>* tsi = sm;
+    cpx<R>* tsf = C::SAME ? sm : sm + C::TWI;
+    PlanTw<N, IP>::fill(tsi, tw, threadIdx.x, C::NT);
+    if constexpr (!C::SAME) PlanTw<N, FP>::fill(tsf, tw, threadIdx.x, C::NT);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int t = lane % TPL, lw = lane / TPL;
+    cpx<R>* lb = sm + C::TWI + C::TWF + (wave * LW + lw) * C::LPL;
+    const size_t line = size_t(blk) * C::LINES + wave * LW + lw;
+    const cpx<R>* src = in + line * KZP;
+    cpx<R>* dst = out + line * KZP;
+
+    cpx<R> v[5][E];
+    zw_load<R, N, C, 0, 1>(v[0], src, NNKZ, t, lane);
+    zw_load<R, N, C, 2, 3>(v[1], src, NNKZ, t, lane);
+    zw_load<R, N, C, 4, 5>(v[2], src, NNKZ, t, lane);
+    zw_load<R, N, C, 6, 7>(v[3], src, NNKZ, t, lane);
+    zw_load<R, N, C, 8, -1>(v[4], src, NNKZ, t, lane);
+    __syncthreads();  // twiddle rows staged
 #pragma unroll
-    for (int j = 0; j < NB1; ++j)
+    for (int p = 0; p < 5; ++p) reg_fft<R, N, E, +1, PS>(v[p], t, lb, tsi, IP{});
+    cpx<R> f[3][E];
 #pragma unroll
-      for (int m = 0; m < R1; ++m) lb[pidx<PS>(t * NB1 + j + m * S1)] = a[j * R1 + m];
-    wave_lds_sync();
-#pragma unroll
-    for (int i = 0; i < (KZ + TPL - 1) / TPL; ++i) {
-      const int kz = t + i * TPL;
-      if (kz < KZ) {
-        const cpx<R> P = lb[pidx<PS>(kz)];
-        const cpx<R> Qc = lb[pidx<PS>(kz == 0 ? 0 : N - kz)];
-        gst<nt_mid<N>()>(&dst[kz], cpx<R>{R(0.5) * (P.x + Qc.x), R(0.5) * (P.y - Qc.y)});
-        gst<nt_mid<N>()>(&dst[NNKZ + kz], cpx<R>{R(0.5) * (P.y + Qc.y), R(0.5) * (Qc.x - P.x)});
-      }
+    for (int i = 0; i < E; ++i) {
+      const R u[3] = {v[0][i].x, v[0][i].y, v[1][i].x};
+      const R w[3] = {v[1][i].y, v[2][i].x, v[2][i].y};
+      const R q[3] = {v[3][i].x, v[3][i].y, v[4][i].x};
+      R p1[3], p2[3];
+      cross3(w, q, p1);
+      cross3(q, u, p2);
+      f[0][i] = {p1[0], p1[1]};
+      f[1][i] = {p1[2], p2[0]};
+      f[2][i] = {p2[1], p2[2]};
     }
+#pragma unroll
+    for (int p = 0; p < 3; ++p) reg_fft<R, N, E, -1, PS>(f[p], t, lb, tsf, FP{});
+    zw_store_pair<R, N, C, 0, 1>(f[0], lb, dst, NNKZ, t);
+    zw_store_pair<R, N, C, 2, 3>(f[1], lb, dst, NNKZ, t);
+    zw_store_pair<R, N, C, 4, 5>(f[2], lb, dst, NNKZ, t);
   }
 };
 
@@ -819,6 +902,16 @@ template <typename R, int N, int E, int WPB>
 __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(zw_waves_per_eu<R, E>())))
 void k_zw(const cpx<R>* __restrict__ in, cpx<R>* __restrict__ out, const cpx<R>* __restrict__ tw) {
   using B = ZWBody<R, N, E, WPB>;
+  __shared__ cpx<R> sm[B::SMEM];
+  B::run(sm, blockIdx.x, in, out, tw);
+}
+
+// Register-resident adjoint z pass: 8 points per lane (five sequences per
+// line), two wavefronts per SIMD.
+template <typename R, int N, int WPB>
+__global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(2)))
+void k_zw_adj(const cpx<R>* __restrict__ in, cpx<R>* __restrict__ out, const cpx<R>* __restrict__ tw) {
+  using B = ZWAdjBody<R, N, 8, WPB>;
   __shared__ cpx<R> sm[B::SMEM];
   B::run(sm, blockIdx.x, in, out, tw);
 }

@@ -188,7 +188,16 @@ struct Launch {
       case ZM_PHYS_OUT3: return z_<ZM_PHYS_OUT3, 0>(in, out, pin, pout, tw, lor, s);
       case ZM_MAXABS: return z_<ZM_MAXABS, 0>(in, out, pin, pout, tw, lor, s);
       case ZM_PHYS_OUT1: return z_<ZM_PHYS_OUT1, 0>(in, out, pin, pout, tw, lor, s);
-      case ZM_ADJ: return z_<ZM_ADJ, 0>(in, out, pin, pout, tw, lor, s);
+      case ZM_ADJ:
+        if constexpr (N >= 32 && (N & (N - 1)) == 0) {
+          constexpr int WPB = ZwDefault<N>::WPB;
+          using C = CfgZW<R, N, 8, WPB>;
+          hipLaunchKernelGGL((k_zw_adj<R, N, WPB>), dim3(N * N / C::LINES), dim3(C::NT), 0, s, in, out, tw);
+          TG_LAUNCHED();
+          return TG_OK;
+        } else {
+          return z_<ZM_ADJ, 0>(in, out, pin, pout, tw, lor, s);
+        }
       default: return z_<ZM_PHYS_IN3, 0>(in, out, pin, pout, tw, lor, s);
     }
   }
@@ -370,10 +379,12 @@ struct Ops {
                         hipStream_t s) {
     using Lc = Launch<R, N>;
     const cpx<R>* tw = static_cast<const cpx<R>*>(p->tw);
-    TG_TIMED(TG_PASS_INV_X, Lc::inv_x(XM_ADJ, 0, x, a.a, w.A, tw, R(p->ks), s, a.alpha));
-    TG_TIMED(TG_PASS_INV_Y, Lc::y(+1, 0, 9, w.A, w.B, tw, s));
+    // the 8-points-per-thread tile (variant 1, where compiled) keeps the six
+    // state and cotangent components in registers
+    TG_TIMED(TG_PASS_INV_X, Lc::inv_x(XM_ADJ, num_variants<N>() > 1 ? 1 : 0, x, a.a, w.A, tw, R(p->ks), s, a.alpha));
+    TG_TIMED(TG_PASS_INV_Y, Lc::y(+1, p->var[TG_PASS_INV_Y], 9, w.A, w.B, tw, s));
     TG_TIMED(TG_PASS_Z, Lc::z(ZM_ADJ, 0, w.B, w.A, nullptr, nullptr, tw, R(0), s));
-    TG_TIMED(TG_PASS_FWD_Y, Lc::y(-1, 0, 6, w.A, w.B, tw, s));
+    TG_TIMED(TG_PASS_FWD_Y, Lc::y(-1, p->var[TG_PASS_FWD_Y], 6, w.A, w.B, tw, s));
     TG_TIMED(TG_PASS_FWD_X, Lc::fwd_x(FM_ADJ, 0, w.B, a, tw, s));
     return TG_OK;
   }
