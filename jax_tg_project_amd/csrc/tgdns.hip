@@ -61,6 +61,10 @@ struct tg_plan {
   std::vector<hipEvent_t> evpool;
   std::vector<int> evpass;   // pass id of each recorded pair
   size_t evused = 0;         // pairs recorded since the last read
+  // adjoint: private stream (default priority) on which the reverse sweep
+  // recomputes the next segment while the current one is reversed
+  hipStream_t aux = nullptr;
+  hipEvent_t adjev[3] = {nullptr, nullptr, nullptr};
 };
 
 namespace {
@@ -397,11 +401,15 @@ struct Ops {
     return vjp_passes(p, w, x, a, s);
   }
   // Adjoint workspace: cur | lamp | lacc | abuf | nck checkpoints | 4 seg
-  // stage states | nu-gradient partials.
+  // stage states | nu-gradient partials, and with more than one segment a
+  // second set for the recompute that overlaps the reverse sweep: cur2 |
+  // 4 seg stage states | RHS workspace.
   struct AdjLayout {
     cpx<R>* cur; cpx<R>* lamp; cpx<R>* lacc; cpx<R>* abuf; cpx<R>* ck; cpx<R>* st; double* nupart;
+    cpx<R>* cur2; cpx<R>* st2; void* ws2;
     size_t vstride; int nck, nblk;
   };
+  static size_t nupart_bytes(const tg_plan* p) { return align256(size_t(p->K) * p->KZ * 8 * sizeof(double)); }
   static AdjLayout adj_layout(const tg_plan* p, int nsteps, int seg, void* aws) {
     AdjLayout L;
     const size_t vb = align256(3 * p->compact * sizeof(cpx<R>));
@@ -414,24 +422,28 @@ struct Ops {
     L.ck = reinterpret_cast<cpx<R>*>(base + 4 * vb);
     L.st = reinterpret_cast<cpx<R>*>(base + (4 + L.nck) * vb);
     L.nupart = reinterpret_cast<double*>(base + (4 + L.nck + 4 * size_t(seg)) * vb);
+    char* second = base + (4 + L.nck + 4 * size_t(seg)) * vb + nupart_bytes(p);
+    L.cur2 = L.nck > 1 ? reinterpret_cast<cpx<R>*>(second) : nullptr;
+    L.st2 = L.nck > 1 ? reinterpret_cast<cpx<R>*>(second + vb) : nullptr;
+    L.ws2 = L.nck > 1 ? static_cast<void*>(second + (1 + 4 * size_t(seg)) * vb) : nullptr;
     L.vstride = vb / sizeof(cpx<R>);
     L.nblk = Launch<R, N>::fwd_x_blocks(0, true);
     return L;
   }
-  // `len` RK4 steps from L.cur keeping the four stage states of step j in
-  // L.st[4j .. 4j+3] (stage 1 = the step's input state).
-  static int steps_keeping_stages(const tg_plan* p, const AdjLayout& L, int len, double dt, double nu, void* ws,
-                                  hipStream_t s) {
+  // `len` RK4 steps from cur keeping the four stage states of step j in
+  // st[4j .. 4j+3] (stage 1 = the step's input state).
+  static int steps_keeping_stages(const tg_plan* p, cpx<R>* cur, cpx<R>* st, size_t vstride, int len, double dt,
+                                  double nu, void* ws, hipStream_t s) {
     Ptrs<R> w = carve<R>(p, ws);
     const size_t bytes = 3 * p->compact * sizeof(cpx<R>);
     for (int j = 0; j < len; ++j) {
-      cpx<R>* S = L.st + size_t(4 * j) * L.vstride;
-      TG_HIP(hipMemcpyAsync(S, L.cur, bytes, hipMemcpyDeviceToDevice, s));
+      cpx<R>* S = st + size_t(4 * j) * vstride;
+      TG_HIP(hipMemcpyAsync(S, cur, bytes, hipMemcpyDeviceToDevice, s));
       for (int stg = 0; stg < 4; ++stg) {
         StageArgs<R> a{};
-        a.u0 = L.cur;
-        a.us = S + size_t(stg) * L.vstride;
-        a.unext = stg == 3 ? L.cur : S + size_t(stg + 1) * L.vstride;
+        a.u0 = cur;
+        a.us = S + size_t(stg) * vstride;
+        a.unext = stg == 3 ? cur : S + size_t(stg + 1) * vstride;
         a.uacc = w.Acc;
         a.dt = R(dt); a.nu = R(nu); a.ks = R(p->ks); a.stage = stg;
         TG_RET(rhs_passes<R, N>(p, w, a.us, nullptr, FM_HYDRO_STAGE, a, R(0), s));
@@ -450,7 +462,7 @@ struct Ops {
       TG_HIP(hipMemcpyAsync(L.ck + c * L.vstride, L.cur, bytes, hipMemcpyDeviceToDevice, s));
       const int len = std::min(seg, nsteps - c * seg);
       if (c < L.nck - 1) TG_RET(rk4(p, L.cur, nullptr, len, dt, nu, 0.0, -1.0, ws, s));
-      else TG_RET(steps_keeping_stages(p, L, len, dt, nu, ws, s));  // the reverse sweep starts here
+      else TG_RET(steps_keeping_stages(p, L.cur, L.st, L.vstride, len, dt, nu, ws, s));  // reverse starts here
     }
     double d[NDIAG + 2];
     TG_RET(diagnostics(p, L.cur, nullptr, 0, d, ws, s));
@@ -475,14 +487,36 @@ struct Ops {
     const R h = R(dt);
     const R cl[4] = {h / R(6), h / R(3), h / R(3), h / R(6)};  // weight of lamp in a_s (s = 1..4)
     const R cm[4] = {R(0), h / R(2), h / R(2), h};              // weight of mu_{s+1} in a_s
+    // The last segment's stages are kept by the forward sweep (buffer 0).
+    // Segment c - 1 is recomputed on the private stream into the other
+    // buffer while segment c is reversed on `s`; events order the buffer
+    // reuse both ways.
+    hipStream_t s2 = nullptr;
+    hipEvent_t ev_in = nullptr, ev_rec = nullptr, ev_rev = nullptr;
+    if (L.nck > 1) {
+      tg_plan* mp = const_cast<tg_plan*>(p);
+      if (!mp->aux) TG_HIP(hipStreamCreateWithFlags(&mp->aux, hipStreamNonBlocking));
+      for (hipEvent_t& e : mp->adjev)
+        if (!e) TG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      s2 = mp->aux;
+      ev_in = mp->adjev[0]; ev_rec = mp->adjev[1]; ev_rev = mp->adjev[2];
+      TG_HIP(hipEventRecord(ev_in, s));  // checkpoints written, nupart cleared
+      TG_HIP(hipStreamWaitEvent(s2, ev_in, 0));
+    }
     for (int c = L.nck - 1; c >= 0; --c) {
       const int len = std::min(seg, nsteps - c * seg);
-      if (c < L.nck - 1) {  // the last segment's stages are kept by the forward sweep
-        TG_HIP(hipMemcpyAsync(L.cur, L.ck + c * L.vstride, bytes, hipMemcpyDeviceToDevice, s));
-        TG_RET(steps_keeping_stages(p, L, len, dt, nu, ws, s));
+      cpx<R>* stc = ((L.nck - 1 - c) % 2 == 0) ? L.st : L.st2;
+      if (c < L.nck - 1) TG_HIP(hipStreamWaitEvent(s, ev_rec, 0));  // segment c recomputed
+      if (c > 0) {
+        // recompute segment c - 1 into the buffer segment c + 1 used
+        cpx<R>* stn = ((L.nck - c) % 2 == 0) ? L.st : L.st2;
+        const int lenn = std::min(seg, nsteps - (c - 1) * seg);
+        TG_HIP(hipMemcpyAsync(L.cur2, L.ck + (c - 1) * L.vstride, bytes, hipMemcpyDeviceToDevice, s2));
+        TG_RET(steps_keeping_stages(p, L.cur2, stn, L.vstride, lenn, dt, nu, L.ws2, s2));
+        TG_HIP(hipEventRecord(ev_rec, s2));
       }
       for (int j = len - 1; j >= 0; --j) {
-        cpx<R>* S = L.st + size_t(4 * j) * L.vstride;
+        cpx<R>* S = stc + size_t(4 * j) * L.vstride;
         for (int sidx = 3; sidx >= 0; --sidx) {
           StageArgs<R> a{};
           a.nu = R(nu); a.ks = R(p->ks);
@@ -495,6 +529,11 @@ struct Ops {
           TG_RET(vjp_passes(p, w, a.xs, a, s));
         }
         std::swap(lamp, lacc);
+      }
+      if (c > 1) {
+        // the recompute of segment c - 2 reuses this buffer: after this reverse
+        TG_HIP(hipEventRecord(ev_rev, s));
+        TG_HIP(hipStreamWaitEvent(s2, ev_rev, 0));
       }
     }
     hipLaunchKernelGGL((k_adj_final<R, N>), dim3(cdiv(int(p->compact), 256)), dim3(256), 0, s, lamp, grad);
@@ -702,6 +741,9 @@ int tg_plan_create(int n, int precision_bits, int mhd, double domain_length, tg_
 int tg_plan_destroy(tg_plan* p) {
   if (!p) return TG_OK;
   for (hipEvent_t e : p->evpool) (void)hipEventDestroy(e);
+  for (hipEvent_t e : p->adjev)
+    if (e) (void)hipEventDestroy(e);
+  if (p->aux) (void)hipStreamDestroy(p->aux);
   if (p->tw) (void)hipFree(p->tw);
   delete p;
   return TG_OK;
@@ -796,7 +838,9 @@ int tg_adjoint_workspace(tg_plan* p, int nsteps, int seg, long long* bytes) {
   if (nsteps < 1 || seg < 1 || !bytes) return fail(TG_ERR_ARG, "tg_adjoint_workspace: need nsteps, seg >= 1");
   const size_t vb = align256(3 * p->compact * p->cb);
   const size_t nck = (nsteps + seg - 1) / seg;
-  *bytes = (long long)((4 + nck + 4 * size_t(seg)) * vb + align256(size_t(p->K) * p->KZ * 8 * sizeof(double)));
+  size_t b = (4 + nck + 4 * size_t(seg)) * vb + align256(size_t(p->K) * p->KZ * 8 * sizeof(double));
+  if (nck > 1) b += (1 + 4 * size_t(seg)) * vb + align256(p->wsBytes);  // overlapped recompute
+  *bytes = (long long)b;
   return TG_OK;
 }
 
