@@ -157,8 +157,9 @@ int tg_batch_query(const tg_batch* batch, long long* state_offsets, long long* w
 /* Record the state and workspace device pointers used by later steps. */
 int tg_batch_bind(tg_batch* batch, void* state, void* ws);
 /* nsteps classic RK4 steps of every instance with time step dt.  With
- * use_graph != 0 one step (4 stages x 5 batched launches) is captured once
- * into a HIP graph (again when dt changes) and replayed nsteps times. */
+ * use_graph != 0 one step (4 stages x 5 batched launches per size class) is
+ * captured once into a HIP graph (again when dt changes), with one parallel
+ * branch per size class, and replayed nsteps times. */
 int tg_batch_rk4_steps(tg_batch* batch, int nsteps, double dt, int use_graph, void* stream);
 
 /* ---- Adjoint (no counterpart in the reference, whose only gradient would be

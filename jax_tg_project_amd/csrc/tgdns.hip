@@ -927,6 +927,8 @@ struct tg_batch {
   size_t cb = 16;
   bool bound = false;
   hipStream_t cap = nullptr;         // private capture stream
+  std::vector<hipStream_t> branch;   // capture streams of size classes 1.. (graph branches)
+  std::vector<hipEvent_t> fork;      // per size class: fork and join events
   hipGraphExec_t exec = nullptr;
   double graph_dt = 0.0;
 };
@@ -974,26 +976,44 @@ int batch_launch_class(const tg_batch* b, int c, int st, double dt, hipStream_t 
   return TG_OK;
 }
 
-// One RK4 step: for each stage, the five passes of every size class.  Size
-// classes are independent, so their launches for one pass are issued back to
-// back before the next pass.
+// The four stages of one RK4 step of size class c on stream s.
 template <typename R>
-int batch_launch_step(tg_batch* b, double dt, hipStream_t s) {
+int batch_class_step(tg_batch* b, int c, double dt, hipStream_t s) {
   for (int st = 0; st < 4; ++st) {
-    for (size_t c = 0; c < b->cls_n.size(); ++c) {
-      int rc = TG_OK;
-      switch (b->cls_n[c]) {
-        case 16: rc = batch_launch_class<R, 16>(b, int(c), st, dt, s); break;
-        case 32: rc = batch_launch_class<R, 32>(b, int(c), st, dt, s); break;
-        case 48: rc = batch_launch_class<R, 48>(b, int(c), st, dt, s); break;
-        case 64: rc = batch_launch_class<R, 64>(b, int(c), st, dt, s); break;
-        case 96: rc = batch_launch_class<R, 96>(b, int(c), st, dt, s); break;
-        case 128: rc = batch_launch_class<R, 128>(b, int(c), st, dt, s); break;
-        default: rc = fail(TG_ERR_UNSUPPORTED, "ensemble size class");
-      }
-      if (rc) return rc;
+    int rc = TG_OK;
+    switch (b->cls_n[c]) {
+      case 16: rc = batch_launch_class<R, 16>(b, c, st, dt, s); break;
+      case 32: rc = batch_launch_class<R, 32>(b, c, st, dt, s); break;
+      case 48: rc = batch_launch_class<R, 48>(b, c, st, dt, s); break;
+      case 64: rc = batch_launch_class<R, 64>(b, c, st, dt, s); break;
+      case 96: rc = batch_launch_class<R, 96>(b, c, st, dt, s); break;
+      case 128: rc = batch_launch_class<R, 128>(b, c, st, dt, s); break;
+      default: rc = fail(TG_ERR_UNSUPPORTED, "ensemble size class");
     }
+    if (rc) return rc;
   }
+  return TG_OK;
+}
+
+// One RK4 step of every instance.  Size classes are independent: with
+// `branches` (the graph capture) class c >= 1 runs on its own stream forked
+// from and joined back to s, so the captured graph holds one parallel
+// branch per class; without, the classes run one after the other on s.
+template <typename R>
+int batch_launch_step(tg_batch* b, double dt, hipStream_t s, bool branches) {
+  const int ncls = int(b->cls_n.size());
+  if (!branches || ncls == 1) {
+    for (int c = 0; c < ncls; ++c) TG_RET(batch_class_step<R>(b, c, dt, s));
+    return TG_OK;
+  }
+  TG_HIP(hipEventRecord(b->fork[0], s));
+  for (int c = 1; c < ncls; ++c) {
+    TG_HIP(hipStreamWaitEvent(b->branch[c - 1], b->fork[0], 0));
+    TG_RET(batch_class_step<R>(b, c, dt, b->branch[c - 1]));
+    TG_HIP(hipEventRecord(b->fork[c], b->branch[c - 1]));
+  }
+  TG_RET(batch_class_step<R>(b, 0, dt, s));
+  for (int c = 1; c < ncls; ++c) TG_HIP(hipStreamWaitEvent(s, b->fork[c], 0));
   return TG_OK;
 }
 
@@ -1121,6 +1141,8 @@ int tg_batch_destroy(tg_batch* b) {
   if (!b) return TG_OK;
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->cap) (void)hipStreamDestroy(b->cap);
+  for (hipStream_t t : b->branch) (void)hipStreamDestroy(t);
+  for (hipEvent_t e : b->fork) (void)hipEventDestroy(e);
   for (void* t : b->tw_by_size)
     if (t) (void)hipFree(t);
   if (b->d_inst) (void)hipFree(b->d_inst);
@@ -1148,11 +1170,12 @@ int tg_batch_rk4_steps(tg_batch* b, int nsteps, double dt, int use_graph, void* 
   if (!b->bound) return fail(TG_ERR_ARG, "tg_batch_rk4_steps: call tg_batch_bind first");
   if (nsteps < 0) return fail(TG_ERR_ARG, "nsteps must be >= 0");
   hipStream_t s = static_cast<hipStream_t>(stream);
-  auto step = [&](hipStream_t st) {
-    return b->prec == 64 ? batch_launch_step<double>(b, dt, st) : batch_launch_step<float>(b, dt, st);
+  auto step = [&](hipStream_t st, bool branches) {
+    return b->prec == 64 ? batch_launch_step<double>(b, dt, st, branches)
+                         : batch_launch_step<float>(b, dt, st, branches);
   };
   if (!use_graph) {
-    for (int i = 0; i < nsteps; ++i) TG_RET(step(s));
+    for (int i = 0; i < nsteps; ++i) TG_RET(step(s, false));
     return TG_OK;
   }
   if (!b->exec || b->graph_dt != dt) {
@@ -1161,9 +1184,20 @@ int tg_batch_rk4_steps(tg_batch* b, int nsteps, double dt, int use_graph, void* 
       b->exec = nullptr;
     }
     if (!b->cap) TG_HIP(hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking));
+    const size_t ncls = b->cls_n.size();
+    while (b->branch.size() + 1 < ncls) {
+      hipStream_t t = nullptr;
+      TG_HIP(hipStreamCreateWithFlags(&t, hipStreamNonBlocking));
+      b->branch.push_back(t);
+    }
+    while (b->fork.size() < ncls) {
+      hipEvent_t e = nullptr;
+      TG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      b->fork.push_back(e);
+    }
     hipGraph_t g = nullptr;
     TG_HIP(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal));
-    const int rc = step(b->cap);
+    const int rc = step(b->cap, true);
     hipError_t e = hipStreamEndCapture(b->cap, &g);
     if (rc) {
       if (g) (void)hipGraphDestroy(g);
