@@ -220,7 +220,10 @@ struct XInvBody {
         }
       }
     }
-#pragma unroll 1
+    // Unrolled: each field's formula is resolved at compile time (a run-time
+    // field index costs a branch chain per point and keeps every candidate's
+    // operands live).
+#pragma unroll
     for (int f = 0; f < F; ++f) {
       if (f) __syncthreads();  // the previous field's last LDS reads are done
       auto ld = [&](int n, int ll, int slot) -> cpx<R> {
