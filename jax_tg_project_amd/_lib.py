@@ -232,10 +232,14 @@ class Plan:
                                     _stream()))
         return out
 
-    def adjoint_workspace(self, nsteps, seg):
+    def adjoint_workspace_bytes(self, nsteps, seg):
+        """Bytes of adjoint workspace for (nsteps, seg)."""
         b = ctypes.c_longlong()
         _check(self._lib.tg_adjoint_workspace(self._h, int(nsteps), int(seg), ctypes.byref(b)))
-        return torch.empty(int(b.value), dtype=torch.uint8, device=self.device)
+        return int(b.value)
+
+    def adjoint_workspace(self, nsteps, seg):
+        return torch.empty(self.adjoint_workspace_bytes(nsteps, seg), dtype=torch.uint8, device=self.device)
 
     def adjoint_forward(self, x0_hat, nsteps, dt, nu, objective, seg, aws):
         self._spec(x0_hat, "x0_hat")

@@ -27,15 +27,14 @@ def sqrt_segment(nsteps):
 
 
 def choose_segment(plan, nsteps, budget_bytes=None):
-    """Checkpoint segment length: every stage state of the whole trajectory
-    (seg = nsteps: nothing recomputed) when 4 * nsteps state vectors fit in
+    """Checkpoint segment length: every stage of the whole trajectory kept
+    (seg = nsteps: nothing recomputed) when that adjoint workspace fits in
     `budget_bytes` (default: a quarter of the free device memory), else the
     sqrt(n) schedule."""
-    vec = 3 * plan.K * plan.K * plan.KZ * (16 if plan.dtype == torch.float64 else 8)
     if budget_bytes is None:
         free, _ = torch.cuda.mem_get_info(plan.device)
         budget_bytes = free // 4
-    return nsteps if 4 * nsteps * vec <= budget_bytes else sqrt_segment(nsteps)
+    return nsteps if plan.adjoint_workspace_bytes(nsteps, nsteps) <= budget_bytes else sqrt_segment(nsteps)
 
 
 class FinalTimeObjective(torch.autograd.Function):

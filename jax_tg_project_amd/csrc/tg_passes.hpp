@@ -82,24 +82,29 @@ struct CfgXF {
   static constexpr int NT = LINES * N / E;
 };
 
-enum XMode { XM_HYDRO = 0, XM_MHD = 1, XM_COPY3 = 2, XM_DIV = 3, XM_CURL3 = 4, XM_ADJ = 5 };
+enum XMode { XM_HYDRO = 0, XM_MHD = 1, XM_COPY3 = 2, XM_DIV = 3, XM_CURL3 = 4, XM_ADJ = 5, XM_ADJQ = 6 };
 enum ZMode { ZM_HYDRO = 0, ZM_MHD = 1, ZM_PHYS_OUT3 = 2, ZM_MAXABS = 3, ZM_PHYS_IN3 = 4, ZM_PHYS_OUT1 = 5, ZM_ADJ = 6 };
 enum FMode { FM_HYDRO_STAGE = 0, FM_MHD_STAGE = 1, FM_STORE3 = 2, FM_HYDRO_RHS = 3, FM_MHD_RHS = 4, FM_ADJ = 5 };
 
 __host__ __device__ constexpr int xmode_fields(int m) {
-  return m == XM_HYDRO ? 6 : m == XM_MHD ? 12 : m == XM_ADJ ? 9 : (m == XM_COPY3 || m == XM_CURL3) ? 3 : 1;
+  return m == XM_HYDRO ? 6 : m == XM_MHD ? 12 : m == XM_ADJ ? 9 : (m == XM_COPY3 || m == XM_CURL3 || m == XM_ADJQ) ? 3 : 1;
 }
 
 // ----------------------------------------------------------------- pass 1
 // Inverse transform along x of the fields derived from the compact state:
 // HYDRO -> (u, omega) = (S, i k x S); MHD -> (u, omega, b, j); COPY3 -> S;
-// CURL3 -> i k x S; DIV -> i k . S.  The 1/N^3 inverse normalisation is
-// applied here.
+// CURL3 -> i k x S; DIV -> i k . S; ADJ -> (u, omega, P(alpha a)) of the
+// state and the cotangent; ADJQ -> P(alpha S) of a cotangent S alone.  The
+// 1/N^3 inverse normalisation is applied here.
 template <typename R, int N, int MODE>
 __device__ __forceinline__ cpx<R> xfield(int f, const cpx<R>* __restrict__ su,
                                          const cpx<R>* __restrict__ sb, size_t pt, size_t cs,
                                          R kx, R ky, R kz, R alpha = R(1)) {
-  if constexpr (MODE == XM_ADJ) {
+  if constexpr (MODE == XM_ADJQ) {  // P(alpha * a) of the cotangent su
+    sb = su;
+    f += 6;
+  }
+  if constexpr (MODE == XM_ADJ || MODE == XM_ADJQ) {
     // fields 0-5: u, curl u of the state su; 6-8: P(alpha * a) of the
     // cotangent sb (Leray projection, k2 == 0 -> 1)
     if (f >= 6) {
@@ -136,9 +141,11 @@ template <typename R, int N, int MODE, int NS, int E>
 __device__ __forceinline__ cpx<R> xfield_reg(int f, const cpx<R> (&sv)[NS][E], int slot, R kx, R ky, R kz,
                                              R alpha) {
   int o = 0;
-  if constexpr (MODE == XM_ADJ) {
-    if (f >= 6) {
-      const cpx<R> a = sv[3][slot], b = sv[4][slot], c = sv[5][slot];
+  if constexpr (MODE == XM_ADJ || MODE == XM_ADJQ) {
+    if (MODE == XM_ADJQ || f >= 6) {
+      constexpr int q0 = MODE == XM_ADJQ ? 0 : 3;  // components of the cotangent
+      if constexpr (MODE == XM_ADJQ) f += 6;
+      const cpx<R> a = sv[q0][slot], b = sv[q0 + 1][slot], c = sv[q0 + 2][slot];
       const R k2 = kx * kx + ky * ky + kz * kz;
       const R k2s = (k2 == R(0)) ? R(1) : k2;
       const cpx<R> kd = {kx * a.x + ky * b.x + kz * c.x, kx * a.y + ky * b.y + kz * c.y};
@@ -827,10 +834,11 @@ struct ZWBody {
 };
 
 // Adjoint z pass with the lines in registers: per line the nine inverse
-// fields u (0-2), w = curl u (3-5) and q (6-8) as five packed sequences
-// (u_x + i u_y), (u_z + i w_x), (w_y + i w_z), (q_x + i q_y), (q_z + 0 i);
-// products p1 = w x q and p2 = q x u; three forward sequences (p1_x + i
-// p1_y), (p1_z + i p2_x), (p2_y + i p2_z) stored as fields 0-5.
+// fields u, w = curl u (fields 0-5 of `in`) and q (fields 0-2 of `qin`) as
+// five packed sequences (u_x + i u_y), (u_z + i w_x), (w_y + i w_z), (q_x +
+// i q_y), (q_z + 0 i); products p1 = w x q and p2 = q x u; three forward
+// sequences (p1_x + i p1_y), (p1_z + i p2_x), (p2_y + i p2_z) stored as
+// fields 0-5.
 template <typename R, int N, int E_, int WPB>
 struct ZWAdjBody {
   using G = Geo<N>;
@@ -839,7 +847,8 @@ struct ZWAdjBody {
   using FP = typename C::FP;
   static constexpr int SMEM = C::SMEM;
   __device__ __forceinline__ static void run(cpx<R>* sm, int blk, const cpx<R>* __restrict__ in,
-                                             cpx<R>* __restrict__ out, const cpx<R>* __restrict__ tw) {
+                                             const cpx<R>* __restrict__ qin, cpx<R>* __restrict__ out,
+                                             const cpx<R>* __restrict__ tw) {
     constexpr int E = C::E, TPL = C::TPL, LW = C::LW, PS = C::PS;
     constexpr int KZP = kzp<R, N>();
     const size_t NNKZ = size_t(N) * N * KZP;
@@ -852,14 +861,15 @@ struct ZWAdjBody {
     cpx<R>* lb = sm + C::TWI + C::TWF + (wave * LW + lw) * C::LPL;
     const size_t line = size_t(blk) * C::LINES + wave * LW + lw;
     const cpx<R>* src = in + line * KZP;
+    const cpx<R>* qsrc = qin + line * KZP;
     cpx<R>* dst = out + line * KZP;
 
     cpx<R> v[5][E];
     zw_load<R, N, C, 0, 1>(v[0], src, NNKZ, t, lane);
     zw_load<R, N, C, 2, 3>(v[1], src, NNKZ, t, lane);
     zw_load<R, N, C, 4, 5>(v[2], src, NNKZ, t, lane);
-    zw_load<R, N, C, 6, 7>(v[3], src, NNKZ, t, lane);
-    zw_load<R, N, C, 8, -1>(v[4], src, NNKZ, t, lane);
+    zw_load<R, N, C, 0, 1>(v[3], qsrc, NNKZ, t, lane);
+    zw_load<R, N, C, 2, -1>(v[4], qsrc, NNKZ, t, lane);
     __syncthreads();  // twiddle rows staged
 #pragma unroll
     for (int p = 0; p < 5; ++p) reg_fft<R, N, E, +1, PS>(v[p], t, lb, tsi, IP{});
@@ -913,10 +923,11 @@ void k_zw(const cpx<R>* __restrict__ in, cpx<R>* __restrict__ out, const cpx<R>*
 // line), two wavefronts per SIMD.
 template <typename R, int N, int WPB>
 __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(2)))
-void k_zw_adj(const cpx<R>* __restrict__ in, cpx<R>* __restrict__ out, const cpx<R>* __restrict__ tw) {
+void k_zw_adj(const cpx<R>* __restrict__ in, const cpx<R>* __restrict__ qin, cpx<R>* __restrict__ out,
+              const cpx<R>* __restrict__ tw) {
   using B = ZWAdjBody<R, N, 8, WPB>;
   __shared__ cpx<R> sm[B::SMEM];
-  B::run(sm, blockIdx.x, in, out, tw);
+  B::run(sm, blockIdx.x, in, qin, out, tw);
 }
 
 // ----------------------------------------------------------------- pass 5

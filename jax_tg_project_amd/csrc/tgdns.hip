@@ -118,6 +118,12 @@ int with_variant(int v, F&& f) {
   }
 }
 
+// Grids whose adjoint z pass is the register-resident kernel; their reverse
+// sweep also keeps each stage's inverse-y output (u, w) for the VJPs.
+template <int N>
+constexpr bool adj_reg_z() { return N >= 32 && (N & (N - 1)) == 0; }
+inline bool adj_reg_z_rt(int n) { return n >= 32 && (n & (n - 1)) == 0; }
+
 template <typename R, int N>
 struct Launch {
   using G = Geo<N>;
@@ -133,6 +139,7 @@ struct Launch {
         case XM_COPY3: hipLaunchKernelGGL((k_inv_x<R, N, XM_COPY3, V>), grid, blk, 0, s, su, sb, out, tw, ks, al, cpz); break;
         case XM_CURL3: hipLaunchKernelGGL((k_inv_x<R, N, XM_CURL3, V>), grid, blk, 0, s, su, sb, out, tw, ks, al, cpz); break;
         case XM_ADJ: hipLaunchKernelGGL((k_inv_x<R, N, XM_ADJ, V>), grid, blk, 0, s, su, sb, out, tw, ks, al, cpz); break;
+        case XM_ADJQ: hipLaunchKernelGGL((k_inv_x<R, N, XM_ADJQ, V>), grid, blk, 0, s, su, sb, out, tw, ks, al, cpz); break;
         default: hipLaunchKernelGGL((k_inv_x<R, N, XM_DIV, V>), grid, blk, 0, s, su, sb, out, tw, ks, al, cpz); break;
       }
       TG_LAUNCHED();
@@ -193,16 +200,25 @@ struct Launch {
       case ZM_MAXABS: return z_<ZM_MAXABS, 0>(in, out, pin, pout, tw, lor, s);
       case ZM_PHYS_OUT1: return z_<ZM_PHYS_OUT1, 0>(in, out, pin, pout, tw, lor, s);
       case ZM_ADJ:
-        if constexpr (N >= 32 && (N & (N - 1)) == 0) {
-          constexpr int WPB = ZwDefault<N>::WPB;
-          using C = CfgZW<R, N, 8, WPB>;
-          hipLaunchKernelGGL((k_zw_adj<R, N, WPB>), dim3(N * N / C::LINES), dim3(C::NT), 0, s, in, out, tw);
-          TG_LAUNCHED();
-          return TG_OK;
+        if constexpr (adj_reg_z<N>()) {
+          return z_adj(in, in + size_t(6) * N * N * kzp<R, N>(), out, tw, s);  // q = fields 6-8 of in
         } else {
           return z_<ZM_ADJ, 0>(in, out, pin, pout, tw, lor, s);
         }
       default: return z_<ZM_PHYS_IN3, 0>(in, out, pin, pout, tw, lor, s);
+    }
+  }
+  // Register-resident adjoint z pass: u, w from fields 0-5 of `in`, q from
+  // fields 0-2 of `qin`.
+  static int z_adj(const cpx<R>* in, const cpx<R>* qin, cpx<R>* out, const cpx<R>* tw, hipStream_t s) {
+    if constexpr (adj_reg_z<N>()) {
+      constexpr int WPB = ZwDefault<N>::WPB;
+      using C = CfgZW<R, N, 8, WPB>;
+      hipLaunchKernelGGL((k_zw_adj<R, N, WPB>), dim3(N * N / C::LINES), dim3(C::NT), 0, s, in, qin, out, tw);
+      TG_LAUNCHED();
+      return TG_OK;
+    } else {
+      return fail(TG_ERR_UNSUPPORTED, "register adjoint z pass: N must be a power of two >= 32");
     }
   }
   static int nmax() { return (N / CfgZM<R, N, ZM_MAXABS, 0>::LZ) * N; }
@@ -289,8 +305,10 @@ int timed_end(const tg_plan* cp, hipStream_t s) {
 // One right-hand side: stage input (us, bs) -> z-pass input in A/B -> the
 // forward-x epilogue described by `mode` / `a`.
 template <typename R, int N>
+// bslot (hydro only): where the inverse-y output (u, w) goes and the z pass
+// reads it, instead of the workspace B (the adjoint keeps it per stage).
 int rhs_passes(const tg_plan* p, const Ptrs<R>& w, const cpx<R>* us, const cpx<R>* bs, int fmode,
-               const StageArgs<R>& a, R lor, hipStream_t s) {
+               const StageArgs<R>& a, R lor, hipStream_t s, cpx<R>* bslot = nullptr) {
   const int cpz = a.cpz ? a.cpz : Geo<N>::KZ;
   using Lc = Launch<R, N>;
   const cpx<R>* tw = static_cast<const cpx<R>*>(p->tw);
@@ -299,8 +317,9 @@ int rhs_passes(const tg_plan* p, const Ptrs<R>& w, const cpx<R>* us, const cpx<R
   const int* v = p->var;
   TG_TIMED(TG_PASS_INV_X, Lc::inv_x(mhd ? XM_MHD : XM_HYDRO, v[TG_PASS_INV_X], us, bs, w.A, tw, R(p->ks), s, R(1),
                                     cpz));
-  TG_TIMED(TG_PASS_INV_Y, Lc::y(+1, v[TG_PASS_INV_Y], fi, w.A, w.B, tw, s));
-  TG_TIMED(TG_PASS_Z, Lc::z(mhd ? ZM_MHD : ZM_HYDRO, v[TG_PASS_Z], w.B, w.A, nullptr, nullptr, tw, lor, s));
+  cpx<R>* bm = bslot ? bslot : w.B;
+  TG_TIMED(TG_PASS_INV_Y, Lc::y(+1, v[TG_PASS_INV_Y], fi, w.A, bm, tw, s));
+  TG_TIMED(TG_PASS_Z, Lc::z(mhd ? ZM_MHD : ZM_HYDRO, v[TG_PASS_Z], bm, w.A, nullptr, nullptr, tw, lor, s));
   TG_TIMED(TG_PASS_FWD_Y, Lc::y(-1, v[TG_PASS_FWD_Y], fo, w.A, w.B, tw, s));
   TG_TIMED(TG_PASS_FWD_X, Lc::fwd_x(fmode, v[TG_PASS_FWD_X], w.B, a, tw, s));
   return TG_OK;
@@ -379,10 +398,20 @@ struct Ops {
   }
   // One vector-Jacobian product of the hydro RHS at x for the cotangent a
   // (scaled by a.alpha), with the epilogue bookkeeping described in `a`.
+  // uw: the stage's kept inverse-y output (u, w) of x, if any: then only the
+  // cotangent's projection q is transformed.
   static int vjp_passes(const tg_plan* p, const Ptrs<R>& w, const cpx<R>* x, const StageArgs<R>& a,
-                        hipStream_t s) {
+                        hipStream_t s, const cpx<R>* uw = nullptr) {
     using Lc = Launch<R, N>;
     const cpx<R>* tw = static_cast<const cpx<R>*>(p->tw);
+    if (uw) {
+      TG_TIMED(TG_PASS_INV_X, Lc::inv_x(XM_ADJQ, p->var[TG_PASS_INV_X], a.a, nullptr, w.A, tw, R(p->ks), s, a.alpha));
+      TG_TIMED(TG_PASS_INV_Y, Lc::y(+1, p->var[TG_PASS_INV_Y], 3, w.A, w.B, tw, s));
+      TG_TIMED(TG_PASS_Z, Lc::z_adj(uw, w.B, w.A, tw, s));
+      TG_TIMED(TG_PASS_FWD_Y, Lc::y(-1, p->var[TG_PASS_FWD_Y], 6, w.A, w.B, tw, s));
+      TG_TIMED(TG_PASS_FWD_X, Lc::fwd_x(FM_ADJ, 0, w.B, a, tw, s));
+      return TG_OK;
+    }
     // the 8-points-per-thread tile (variant 1, where compiled) keeps the six
     // state and cotangent components in registers
     TG_TIMED(TG_PASS_INV_X, Lc::inv_x(XM_ADJ, num_variants<N>() > 1 ? 1 : 0, x, a.a, w.A, tw, R(p->ks), s, a.alpha));
@@ -404,11 +433,15 @@ struct Ops {
   // stage states | nu-gradient partials, and with more than one segment a
   // second set for the recompute that overlaps the reverse sweep: cur2 |
   // 4 seg stage states | RHS workspace.
+  // On grids with the register adjoint z pass, each stage buffer has a
+  // matching set of 4 seg slots for the stages' inverse-y output (u, w).
   struct AdjLayout {
     cpx<R>* cur; cpx<R>* lamp; cpx<R>* lacc; cpx<R>* abuf; cpx<R>* ck; cpx<R>* st; double* nupart;
     cpx<R>* cur2; cpx<R>* st2; void* ws2;
-    size_t vstride; int nck, nblk;
+    cpx<R>* slots; cpx<R>* slots2;
+    size_t vstride, sstride; int nck, nblk;
   };
+  static size_t slot_bytes(const tg_plan* p) { return align256(size_t(6) * N * N * kzp<R, N>() * sizeof(cpx<R>)); }
   static size_t nupart_bytes(const tg_plan* p) { return align256(size_t(p->K) * p->KZ * 8 * sizeof(double)); }
   static AdjLayout adj_layout(const tg_plan* p, int nsteps, int seg, void* aws) {
     AdjLayout L;
@@ -426,6 +459,11 @@ struct Ops {
     L.cur2 = L.nck > 1 ? reinterpret_cast<cpx<R>*>(second) : nullptr;
     L.st2 = L.nck > 1 ? reinterpret_cast<cpx<R>*>(second + vb) : nullptr;
     L.ws2 = L.nck > 1 ? static_cast<void*>(second + (1 + 4 * size_t(seg)) * vb) : nullptr;
+    char* third = L.nck > 1 ? second + (1 + 4 * size_t(seg)) * vb + align256(p->wsBytes) : second;
+    const size_t sb = slot_bytes(p);
+    L.slots = adj_reg_z<N>() ? reinterpret_cast<cpx<R>*>(third) : nullptr;
+    L.slots2 = (adj_reg_z<N>() && L.nck > 1) ? reinterpret_cast<cpx<R>*>(third + 4 * size_t(seg) * sb) : nullptr;
+    L.sstride = sb / sizeof(cpx<R>);
     L.vstride = vb / sizeof(cpx<R>);
     L.nblk = Launch<R, N>::fwd_x_blocks(0, true);
     return L;
@@ -433,7 +471,8 @@ struct Ops {
   // `len` RK4 steps from cur keeping the four stage states of step j in
   // st[4j .. 4j+3] (stage 1 = the step's input state).
   static int steps_keeping_stages(const tg_plan* p, cpx<R>* cur, cpx<R>* st, size_t vstride, int len, double dt,
-                                  double nu, void* ws, hipStream_t s) {
+                                  double nu, void* ws, hipStream_t s, cpx<R>* slots = nullptr,
+                                  size_t sstride = 0) {
     Ptrs<R> w = carve<R>(p, ws);
     const size_t bytes = 3 * p->compact * sizeof(cpx<R>);
     for (int j = 0; j < len; ++j) {
@@ -446,7 +485,8 @@ struct Ops {
         a.unext = stg == 3 ? cur : S + size_t(stg + 1) * vstride;
         a.uacc = w.Acc;
         a.dt = R(dt); a.nu = R(nu); a.ks = R(p->ks); a.stage = stg;
-        TG_RET(rhs_passes<R, N>(p, w, a.us, nullptr, FM_HYDRO_STAGE, a, R(0), s));
+        cpx<R>* slot = slots ? slots + size_t(4 * j + stg) * sstride : nullptr;
+        TG_RET(rhs_passes<R, N>(p, w, a.us, nullptr, FM_HYDRO_STAGE, a, R(0), s, slot));
       }
     }
     return TG_OK;
@@ -462,7 +502,8 @@ struct Ops {
       TG_HIP(hipMemcpyAsync(L.ck + c * L.vstride, L.cur, bytes, hipMemcpyDeviceToDevice, s));
       const int len = std::min(seg, nsteps - c * seg);
       if (c < L.nck - 1) TG_RET(rk4(p, L.cur, nullptr, len, dt, nu, 0.0, -1.0, ws, s));
-      else TG_RET(steps_keeping_stages(p, L.cur, L.st, L.vstride, len, dt, nu, ws, s));  // reverse starts here
+      else TG_RET(steps_keeping_stages(p, L.cur, L.st, L.vstride, len, dt, nu, ws, s, L.slots,
+                                       L.sstride));  // the reverse sweep starts here
     }
     double d[NDIAG + 2];
     TG_RET(diagnostics(p, L.cur, nullptr, 0, d, ws, s));
@@ -505,14 +546,18 @@ struct Ops {
     }
     for (int c = L.nck - 1; c >= 0; --c) {
       const int len = std::min(seg, nsteps - c * seg);
-      cpx<R>* stc = ((L.nck - 1 - c) % 2 == 0) ? L.st : L.st2;
+      const bool even = (L.nck - 1 - c) % 2 == 0;
+      cpx<R>* stc = even ? L.st : L.st2;
+      cpx<R>* slc = even ? L.slots : L.slots2;
       if (c < L.nck - 1) TG_HIP(hipStreamWaitEvent(s, ev_rec, 0));  // segment c recomputed
       if (c > 0) {
         // recompute segment c - 1 into the buffer segment c + 1 used
-        cpx<R>* stn = ((L.nck - c) % 2 == 0) ? L.st : L.st2;
+        const bool evn = (L.nck - c) % 2 == 0;
+        cpx<R>* stn = evn ? L.st : L.st2;
+        cpx<R>* sln = evn ? L.slots : L.slots2;
         const int lenn = std::min(seg, nsteps - (c - 1) * seg);
         TG_HIP(hipMemcpyAsync(L.cur2, L.ck + (c - 1) * L.vstride, bytes, hipMemcpyDeviceToDevice, s2));
-        TG_RET(steps_keeping_stages(p, L.cur2, stn, L.vstride, lenn, dt, nu, L.ws2, s2));
+        TG_RET(steps_keeping_stages(p, L.cur2, stn, L.vstride, lenn, dt, nu, L.ws2, s2, sln, L.sstride));
         TG_HIP(hipEventRecord(ev_rec, s2));
       }
       for (int j = len - 1; j >= 0; --j) {
@@ -526,7 +571,8 @@ struct Ops {
           else { a.a = L.abuf; a.alpha = R(1); a.first = 0; }
           a.last = sidx == 0;
           if (sidx > 0) { a.anext = L.abuf; a.clam = cl[sidx - 1]; a.cmu = cm[sidx]; }
-          TG_RET(vjp_passes(p, w, a.xs, a, s));
+          const cpx<R>* uw = slc ? slc + size_t(4 * j + sidx) * L.sstride : nullptr;
+          TG_RET(vjp_passes(p, w, a.xs, a, s, uw));
         }
         std::swap(lamp, lacc);
       }
@@ -840,6 +886,11 @@ int tg_adjoint_workspace(tg_plan* p, int nsteps, int seg, long long* bytes) {
   const size_t nck = (nsteps + seg - 1) / seg;
   size_t b = (4 + nck + 4 * size_t(seg)) * vb + align256(size_t(p->K) * p->KZ * 8 * sizeof(double));
   if (nck > 1) b += (1 + 4 * size_t(seg)) * vb + align256(p->wsBytes);  // overlapped recompute
+  if (adj_reg_z_rt(p->n)) {  // kept inverse-y output (u, w) per stage, per stage buffer
+    const size_t kzpad = p->prec == 64 ? size_t(kzp_rt<double>(p->n)) : size_t(kzp_rt<float>(p->n));
+    const size_t sb = align256(6 * size_t(p->n) * p->n * kzpad * p->cb);
+    b += (nck > 1 ? 2 : 1) * 4 * size_t(seg) * sb;
+  }
   *bytes = (long long)b;
   return TG_OK;
 }
