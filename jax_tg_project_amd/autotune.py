@@ -64,9 +64,11 @@ def _probe_state(plan):
     return uh, bh
 
 
-def tune(plan, steps=3, use_cache=True, verbose=False):
+def tune(plan, steps=3, use_cache=True, verbose=False, rounds=3):
     """Pick the fastest tile variant per pass for `plan`; returns
-    (variants, {pass: [ms per launch for each variant]})."""
+    (variants, {pass: [ms per launch for each variant]}).  The variants are
+    timed in `rounds` interleaved rounds of `steps` RK4 steps each; a pass
+    keeps its best per-launch average over the rounds."""
     count = plan.variant_count()
     if count == 1:
         return plan.get_variants(), {}
@@ -78,16 +80,18 @@ def tune(plan, steps=3, use_cache=True, verbose=False):
     uh, bh = _probe_state(plan)
     ms = {p: [math.inf] * count for p in PASS_NAMES}
     dt, nu = 1e-4, 1e-3
-    for v in range(count):
-        plan.set_variants([v] * len(PASS_NAMES))
-        plan.rk4(uh, bh, nsteps=1, dt=dt, nu=nu, eta=nu)  # warm-up (code object load)
-        plan.timing(True)
-        plan.rk4(uh, bh, nsteps=steps, dt=dt, nu=nu, eta=nu)
-        t = plan.timing_read()
-        plan.timing(False)
-        for p, (tot, n) in t.items():
-            if n:
-                ms[p][v] = tot / n
+    for r in range(rounds):
+        for v in range(count):
+            plan.set_variants([v] * len(PASS_NAMES))
+            if r == 0:
+                plan.rk4(uh, bh, nsteps=1, dt=dt, nu=nu, eta=nu)  # warm-up (code object load)
+            plan.timing(True)
+            plan.rk4(uh, bh, nsteps=steps, dt=dt, nu=nu, eta=nu)
+            t = plan.timing_read()
+            plan.timing(False)
+            for p, (tot, n) in t.items():
+                if n:
+                    ms[p][v] = min(ms[p][v], tot / n)
     best = [min(range(count), key=lambda v: ms[p][v]) for p in PASS_NAMES]
     plan.set_variants(best)
     if verbose:
