@@ -904,11 +904,11 @@ struct ZwDefault {
 
 // Occupancy target (wavefronts per SIMD, i.e. a register budget of 512 /
 // target per lane): FP64 with 16 points per lane keeps 192 registers of
-// line data alone; the 12-point plans (radix 3, no mirrored loads) need the
-// larger budget in FP32 too.
+// line data alone; in FP32 the 12- and 16-point plans spill under a
+// 128-register budget.
 template <typename R, int E>
 constexpr int zw_waves_per_eu() {
-  return sizeof(R) == 8 ? (E >= 12 ? 2 : 3) : (E == 12 ? 3 : 4);
+  return sizeof(R) == 8 ? (E >= 12 ? 2 : 3) : (E >= 12 ? 3 : 4);
 }
 
 template <typename R, int N, int E, int WPB>
