@@ -894,6 +894,68 @@ struct ZWAdjBody {
   }
 };
 
+// MHD z pass with the lines in registers: per line the twelve inverse fields
+// u (0-2), w = curl u (3-5), b (6-8), j = curl b (9-11) as six packed
+// sequences u_p + i w_p and b_p + i j_p; products m = u x w + lorentz j x b
+// and e = u x b; three forward sequences (m_x + i m_y), (m_z + i e_x),
+// (e_y + i e_z) stored as fields 0-5.
+template <typename R, int N, int E_, int WPB>
+struct ZWMhdBody {
+  using G = Geo<N>;
+  using C = CfgZW<R, N, E_, WPB>;
+  using IP = typename C::IP;
+  using FP = typename C::FP;
+  static constexpr int SMEM = C::SMEM;
+  __device__ __forceinline__ static void run(cpx<R>* sm, int blk, const cpx<R>* __restrict__ in,
+                                             cpx<R>* __restrict__ out, const cpx<R>* __restrict__ tw, R lorentz) {
+    constexpr int E = C::E, TPL = C::TPL, LW = C::LW, PS = C::PS;
+    constexpr int KZP = kzp<R, N>();
+    const size_t NNKZ = size_t(N) * N * KZP;
+    cpx<R>* tsi = sm;
+    cpx<R>* tsf = C::SAME ? sm : sm + C::TWI;
+    PlanTw<N, IP>::fill(tsi, tw, threadIdx.x, C::NT);
+    if constexpr (!C::SAME) PlanTw<N, FP>::fill(tsf, tw, threadIdx.x, C::NT);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int t = lane % TPL, lw = lane / TPL;
+    cpx<R>* lb = sm + C::TWI + C::TWF + (wave * LW + lw) * C::LPL;
+    const size_t line = size_t(blk) * C::LINES + wave * LW + lw;
+    const cpx<R>* src = in + line * KZP;
+    cpx<R>* dst = out + line * KZP;
+
+    cpx<R> v[6][E];
+    zw_load<R, N, C, 0, 3>(v[0], src, NNKZ, t, lane);
+    zw_load<R, N, C, 1, 4>(v[1], src, NNKZ, t, lane);
+    zw_load<R, N, C, 2, 5>(v[2], src, NNKZ, t, lane);
+    zw_load<R, N, C, 6, 9>(v[3], src, NNKZ, t, lane);
+    zw_load<R, N, C, 7, 10>(v[4], src, NNKZ, t, lane);
+    zw_load<R, N, C, 8, 11>(v[5], src, NNKZ, t, lane);
+    __syncthreads();  // twiddle rows staged
+#pragma unroll
+    for (int p = 0; p < 6; ++p) reg_fft<R, N, E, +1, PS>(v[p], t, lb, tsi, IP{});
+    cpx<R> f[3][E];
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      const R u[3] = {v[0][i].x, v[1][i].x, v[2][i].x};
+      const R w[3] = {v[0][i].y, v[1][i].y, v[2][i].y};
+      const R b[3] = {v[3][i].x, v[4][i].x, v[5][i].x};
+      const R j[3] = {v[3][i].y, v[4][i].y, v[5][i].y};
+      R uw[3], jb[3], e[3];
+      cross3(u, w, uw);
+      cross3(j, b, jb);
+      cross3(u, b, e);
+      const R m[3] = {uw[0] + lorentz * jb[0], uw[1] + lorentz * jb[1], uw[2] + lorentz * jb[2]};
+      f[0][i] = {m[0], m[1]};
+      f[1][i] = {m[2], e[0]};
+      f[2][i] = {e[1], e[2]};
+    }
+#pragma unroll
+    for (int p = 0; p < 3; ++p) reg_fft<R, N, E, -1, PS>(f[p], t, lb, tsf, FP{});
+    zw_store_pair<R, N, C, 0, 1>(f[0], lb, dst, NNKZ, t);
+    zw_store_pair<R, N, C, 2, 3>(f[1], lb, dst, NNKZ, t);
+    zw_store_pair<R, N, C, 4, 5>(f[2], lb, dst, NNKZ, t);
+  }
+};
+
 // Default register z-pass shape of a grid size: points per lane and
 // wavefronts per block (E = 0: the grid uses the LDS-resident kernel).
 template <int N>
@@ -917,6 +979,15 @@ void k_zw(const cpx<R>* __restrict__ in, cpx<R>* __restrict__ out, const cpx<R>*
   using B = ZWBody<R, N, E, WPB>;
   __shared__ cpx<R> sm[B::SMEM];
   B::run(sm, blockIdx.x, in, out, tw);
+}
+
+// Register-resident MHD z pass: 8 points per lane (six sequences per line).
+template <typename R, int N, int WPB>
+__global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 8 ? 2 : 3)))
+void k_zw_mhd(const cpx<R>* __restrict__ in, cpx<R>* __restrict__ out, const cpx<R>* __restrict__ tw, R lorentz) {
+  using B = ZWMhdBody<R, N, 8, WPB>;
+  __shared__ cpx<R> sm[B::SMEM];
+  B::run(sm, blockIdx.x, in, out, tw, lorentz);
 }
 
 // Register-resident adjoint z pass: 8 points per lane (five sequences per
@@ -1146,11 +1217,12 @@ struct CfgXR {
   static constexpr int NT = L * N / E;
 };
 
-template <typename R, int N>
+template <typename R, int N, bool MHD = false>
 struct XRBody {
   using G = Geo<N>;
   using C = CfgXR<R, N>;
   static constexpr int SMEM = N * C::L + N;
+  static constexpr int FF = MHD ? 6 : 3;  // fields: n (hydro); m, e (MHD)
   __device__ __forceinline__ static void run(cpx<R>* sm, int bx, int by, const cpx<R>* __restrict__ in,
                                              const StageArgs<R>& a, const cpx<R>* __restrict__ tw) {
     constexpr int L = C::L, E = C::E;
@@ -1164,9 +1236,9 @@ struct XRBody {
     const bool colok = izr < G::KZ;
     const int iz = colok ? izr : G::KZ - 1;
     stage_twiddles<R, N>(tws, tw);
-    cpx<R> o[3][E];
+    cpx<R> o[FF][E];
 #pragma unroll
-    for (int f = 0; f < 3; ++f) {
+    for (int f = 0; f < FF; ++f) {
       if (f) __syncthreads();  // the previous field's last LDS reads are done
       auto ld = [&](int n, int, int) -> cpx<R> {
         const cpx<R> v = gld<nt_mid<N>()>(&in[((size_t(f) * N + n) * G::K + iy) * kzp<R, N>() + iz]);
@@ -1192,6 +1264,7 @@ struct XRBody {
         const R kx = a.ks * R(G::kint(ix));
         const R k2 = kx * kx + ky * ky + kz * kz;
         const R k2s = (k2 == R(0)) ? R(1) : k2;
+        // momentum: r_u = P(n or m) - nu k2 u_s
         project(kx, ky, kz, k2s, v);
         cpx<R> ru[3];
         const R nk = a.nu * k2;
@@ -1201,15 +1274,32 @@ struct XRBody {
           ru[c] = {v[c].x - nk * s.x, v[c].y - nk * s.y};
         }
         rk4_update(a.stage, a.dt, pt, cs, ru, a.u0, a.unext, a.uacc);
+        if constexpr (MHD) {
+          // induction: r_b = P(i k x e) - eta k2 b_s
+          const cpx<R> e0 = o[3][j * RL + m], e1 = o[4][j * RL + m], e2 = o[5][j * RL + m];
+          cpx<R> cb[3];
+          cb[0] = cmuli(cpx<R>{ky * e2.x - kz * e1.x, ky * e2.y - kz * e1.y});
+          cb[1] = cmuli(cpx<R>{kz * e0.x - kx * e2.x, kz * e0.y - kx * e2.y});
+          cb[2] = cmuli(cpx<R>{kx * e1.x - ky * e0.x, kx * e1.y - ky * e0.y});
+          project(kx, ky, kz, k2s, cb);
+          const R ek = a.eta * k2;
+          cpx<R> rb[3];
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            const cpx<R> s = a.bs[c * cs + pt];
+            rb[c] = {cb[c].x - ek * s.x, cb[c].y - ek * s.y};
+          }
+          rk4_update(a.stage, a.dt, pt, cs, rb, a.b0, a.bnext, a.bacc);
+        }
       }
     }
   }
 };
 
-template <typename R, int N>
+template <typename R, int N, bool MHD>
 __global__ __launch_bounds__((CfgXR<R, N>::NT)) void k_fwd_xr(const cpx<R>* __restrict__ in, StageArgs<R> a,
                                                              const cpx<R>* __restrict__ tw) {
-  using B = XRBody<R, N>;
+  using B = XRBody<R, N, MHD>;
   __shared__ cpx<R> sm[B::SMEM];
   B::run(sm, blockIdx.x, blockIdx.y, in, a, tw);
 }

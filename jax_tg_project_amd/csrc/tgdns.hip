@@ -193,6 +193,16 @@ struct Launch {
           return z_<ZM_HYDRO, 0>(in, out, pin, pout, tw, lor, s);
         }
       case ZM_MHD:
+        // variant 0 of the power-of-two grids >= 32 is the register-resident kernel
+        if constexpr (adj_reg_z<N>()) {
+          if (var == 0) {
+            constexpr int WPB = ZwDefault<N>::WPB;
+            using C = CfgZW<R, N, 8, WPB>;
+            hipLaunchKernelGGL((k_zw_mhd<R, N, WPB>), dim3(N * N / C::LINES), dim3(C::NT), 0, s, in, out, tw, lor);
+            TG_LAUNCHED();
+            return TG_OK;
+          }
+        }
         return with_variant<N>(var, [&](auto vc) {
           return z_<ZM_MHD, decltype(vc)::value>(in, out, pin, pout, tw, lor, s);
         });
@@ -222,12 +232,14 @@ struct Launch {
     }
   }
   static int nmax() { return (N / CfgZM<R, N, ZM_MAXABS, 0>::LZ) * N; }
-  // Hydro RK4 stage variant 1 of N >= 128 is the register-epilogue kernel.
+  // RK4 stage variant 1 of N >= 128 (hydro and MHD) is the register-epilogue kernel.
   static int fwd_x(int mode, int var, const cpx<R>* in, const StageArgs<R>& a, const cpx<R>* tw, hipStream_t s) {
     if constexpr (N >= 128) {
-      if (mode == FM_HYDRO_STAGE && var == 1) {
+      if ((mode == FM_HYDRO_STAGE || mode == FM_MHD_STAGE) && var == 1) {
         using C = CfgXR<R, N>;
-        hipLaunchKernelGGL((k_fwd_xr<R, N>), dim3(cdiv(G::KZ, C::L), G::K), dim3(C::NT), 0, s, in, a, tw);
+        const dim3 grid(cdiv(G::KZ, C::L), G::K), blk(C::NT);
+        if (mode == FM_HYDRO_STAGE) hipLaunchKernelGGL((k_fwd_xr<R, N, false>), grid, blk, 0, s, in, a, tw);
+        else hipLaunchKernelGGL((k_fwd_xr<R, N, true>), grid, blk, 0, s, in, a, tw);
         TG_LAUNCHED();
         return TG_OK;
       }
