@@ -1,5 +1,9 @@
-// Stockham FFT engine for gfx950: batches of lines held in LDS, one radix-R
-// butterfly group per thread per stage, twiddles from a host-built table.
+// Stockham FFT engine for gfx950, in two forms: batches of lines held in LDS
+// and shared by the wavefronts of a block (the x and y passes), and lines
+// held in the registers of the lanes of one wavefront with a wavefront-
+// private LDS exchange buffer (the z passes, "register" section below).  One
+// radix-R butterfly group per thread per stage, twiddles from a host-built
+// table.
 //
 // Layout convention: element n of line l lives at buf[n * LP + l].  A thread
 // owns line l = tid % L and slice t = tid / L of that line; in every stage it
@@ -206,59 +210,6 @@ TG_PLAN(128, 16, 16, 8)
 TG_PLAN(256, 8, 8, 8, 4)
 TG_PLAN(256, 16, 16, 16)
 #undef TG_PLAN
-
-// One Stockham stage of radix RAD after P points have been combined.
-template <typename R, int N, int E, int DIR, int P, int RAD>
-__device__ __forceinline__ void stockham_stage(cpx<R>* buf, int LP, int l, int t, bool active,
-                                               const cpx<R>* __restrict__ tw) {
-  static_assert(E % RAD == 0, "radix must divide points per thread");
-  constexpr int NB = E / RAD;
-  constexpr int SIN = N / RAD;
-  cpx<R> v[NB][RAD];
-  if (active) {
-#pragma unroll
-    for (int j = 0; j < NB; ++j) {
-      const int b = t * NB + j;
-#pragma unroll
-      for (int m = 0; m < RAD; ++m) v[j][m] = buf[(b + m * SIN) * LP + l];
-    }
-  }
-  __syncthreads();
-  if (active) {
-#pragma unroll
-    for (int j = 0; j < NB; ++j) {
-      const int b = t * NB + j;
-      const int k = b % P;
-      if constexpr (P > 1) {
-#pragma unroll
-        for (int m = 1; m < RAD; ++m) {
-          cpx<R> w = tw[m * k * (N / (P * RAD))];
-          if constexpr (DIR > 0) w.y = -w.y;
-          v[j][m] = cmul(v[j][m], w);
-        }
-      }
-      dft<RAD, DIR>(v[j]);
-      const int base = (b - k) * RAD + k;
-#pragma unroll
-      for (int m = 0; m < RAD; ++m) buf[(base + m * P) * LP + l] = v[j][m];
-    }
-  }
-  __syncthreads();
-}
-
-template <typename R, int N, int E, int DIR, int P, int R0, int... Rest>
-__device__ __forceinline__ void run_stages(cpx<R>* buf, int LP, int l, int t, bool active,
-                                           const cpx<R>* __restrict__ tw) {
-  stockham_stage<R, N, E, DIR, P, R0>(buf, LP, l, t, active, tw);
-  if constexpr (sizeof...(Rest) > 0)
-    run_stages<R, N, E, DIR, P * R0, Rest...>(buf, LP, l, t, active, tw);
-}
-
-template <typename R, int N, int E, int DIR, int... Rs>
-__device__ __forceinline__ void fft_apply(cpx<R>* buf, int LP, int l, int t, bool active,
-                                          const cpx<R>* __restrict__ tw, Radices<Rs...>) {
-  run_stages<R, N, E, DIR, 1, Rs...>(buf, LP, l, t, active, tw);
-}
 
 // ---------------------------------------------------------------- global I/O
 // Stage with optional direct I/O.  SRC: 0 = LDS, 1 = functor (no LDS read, so
@@ -528,19 +479,6 @@ __device__ __forceinline__ void rs_stages(cpx<R>* v, int t, cpx<R>* lb, const cp
 template <typename R, int N, int E, int DIR, int PS, int... Rs>
 __device__ __forceinline__ void reg_fft(cpx<R>* v, int t, cpx<R>* lb, const cpx<R>* ts, Radices<Rs...>) {
   rs_stages<R, N, E, DIR, PS, 1, Rs...>(v, t, lb, ts);
-}
-
-// Transform `nlines` lines of length N held in LDS (buf[n*LP + l]).  The
-// calling block must have at least nlines * N / E threads; the rest idle but
-// take part in the barriers.  Entry and exit are barrier-separated.
-template <typename R, int N, int E, int DIR>
-__device__ __forceinline__ void fft_lines(cpx<R>* buf, int LP, int nlines,
-                                          const cpx<R>* __restrict__ tw) {
-  const int tid = threadIdx.x;
-  const int l = tid % nlines;
-  const int t = tid / nlines;
-  const bool active = t < N / E;
-  fft_apply<R, N, E, DIR>(buf, LP, l, t, active, tw, typename PlanFor<N, E>::type{});
 }
 
 }  // namespace tg
