@@ -168,12 +168,45 @@ __device__ __forceinline__ void dft16(cpx<R>* v) {
   for (int i = 0; i < 16; ++i) v[i] = o[i];
 }
 
+template <int DIR, typename R>
+__device__ __forceinline__ void dft12(cpx<R>* v) {
+  // n = 4 n1 + n2, k = k1 + 3 k2: DFT3 over n1, twiddle w12^(n2 k1), DFT4
+  // over n2.
+  const R h = R(0.86602540378443864676372317075294);  // sqrt(3)/2
+#pragma unroll
+  for (int n2 = 0; n2 < 4; ++n2) {
+    cpx<R> t[3] = {v[n2], v[4 + n2], v[8 + n2]};
+    dft3<DIR>(t);
+    v[n2] = t[0]; v[4 + n2] = t[1]; v[8 + n2] = t[2];
+  }
+  // now v[4*k1 + n2] holds A[n2][k1]; w12^m = (cos, DIR sin) of m*30 degrees
+  auto rot = [&](cpx<R> a, R c, R sn) -> cpx<R> {
+    return {a.x * c - a.y * R(DIR) * sn, a.x * R(DIR) * sn + a.y * c};
+  };
+  v[5] = rot(v[5], h, R(0.5));        // w^1
+  v[6] = rot(v[6], R(0.5), h);        // w^2
+  v[7] = rot4<DIR>(v[7]);             // w^3
+  v[9] = rot(v[9], R(0.5), h);        // w^2
+  v[10] = rot(v[10], R(-0.5), h);     // w^4
+  v[11] = {-v[11].x, -v[11].y};       // w^6
+  cpx<R> o[12];
+#pragma unroll
+  for (int k1 = 0; k1 < 3; ++k1) {
+    cpx<R> a0 = v[4 * k1 + 0], a1 = v[4 * k1 + 1], a2 = v[4 * k1 + 2], a3 = v[4 * k1 + 3];
+    dft4s<DIR>(a0, a1, a2, a3);
+    o[k1 + 0] = a0; o[k1 + 3] = a1; o[k1 + 6] = a2; o[k1 + 9] = a3;
+  }
+#pragma unroll
+  for (int i = 0; i < 12; ++i) v[i] = o[i];
+}
+
 template <int RAD, int DIR, typename R>
 __device__ __forceinline__ void dft(cpx<R>* v) {
   if constexpr (RAD == 2) dft2<DIR>(v);
   else if constexpr (RAD == 3) dft3<DIR>(v);
   else if constexpr (RAD == 4) dft4<DIR>(v);
   else if constexpr (RAD == 8) dft8<DIR>(v);
+  else if constexpr (RAD == 12) dft12<DIR>(v);
   else if constexpr (RAD == 16) dft16<DIR>(v);
   else static_assert(RAD == 2, "unsupported radix");
 }
@@ -198,12 +231,12 @@ TG_PLAN(16, 16, 16)
 TG_PLAN(32, 4, 4, 4, 2)
 TG_PLAN(32, 8, 8, 4)
 TG_PLAN(32, 16, 16, 2)
-TG_PLAN(48, 12, 4, 4, 3)
+TG_PLAN(48, 12, 12, 4)
 TG_PLAN(48, 24, 8, 3, 2)
 TG_PLAN(64, 4, 4, 4, 4)
 TG_PLAN(64, 8, 8, 8)
 TG_PLAN(64, 16, 16, 4)
-TG_PLAN(96, 12, 4, 4, 2, 3)
+TG_PLAN(96, 12, 12, 4, 2)
 TG_PLAN(96, 24, 8, 4, 3)
 TG_PLAN(128, 8, 8, 8, 2)
 TG_PLAN(128, 16, 16, 8)

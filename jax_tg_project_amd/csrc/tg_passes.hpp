@@ -645,8 +645,8 @@ template <> struct ZWPlan<128, 8> { using type = Radices<8, 4, 4>; };
 template <> struct ZWPlan<64, 16> { using type = Radices<16, 4>; };
 template <> struct ZWPlan<64, 8> { using type = Radices<8, 8>; };
 template <> struct ZWPlan<32, 8> { using type = Radices<8, 4>; };
-template <> struct ZWPlan<96, 12> { using type = Radices<4, 4, 2, 3>; };
-template <> struct ZWPlan<48, 12> { using type = Radices<4, 4, 3>; };
+template <> struct ZWPlan<96, 12> { using type = Radices<12, 4, 2>; };
+template <> struct ZWPlan<48, 12> { using type = Radices<12, 4>; };
 
 template <typename R, int N, int E_, int WPB>
 struct CfgZW {
