@@ -1033,12 +1033,23 @@ struct StageArgs {
   double* nupart;
 };
 
+// Leray projection n - k (k.n) / k2 (k2s: k2 with 0 -> 1).  FP64 divides
+// each k_c (k.n) by k2 as the reference writes it; FP32, whose passes are
+// bound by instruction issue, multiplies by one reciprocal instead (a last-
+// place difference, far inside its error budget).
 template <typename R>
 __device__ __forceinline__ void project(R kx, R ky, R kz, R k2s, cpx<R>* n) {
   const cpx<R> kd = {kx * n[0].x + ky * n[1].x + kz * n[2].x, kx * n[0].y + ky * n[1].y + kz * n[2].y};
-  n[0] = {n[0].x - kx * kd.x / k2s, n[0].y - kx * kd.y / k2s};
-  n[1] = {n[1].x - ky * kd.x / k2s, n[1].y - ky * kd.y / k2s};
-  n[2] = {n[2].x - kz * kd.x / k2s, n[2].y - kz * kd.y / k2s};
+  if constexpr (sizeof(R) == 4) {
+    const R q = R(1) / k2s;
+    n[0] = {n[0].x - kx * kd.x * q, n[0].y - kx * kd.y * q};
+    n[1] = {n[1].x - ky * kd.x * q, n[1].y - ky * kd.y * q};
+    n[2] = {n[2].x - kz * kd.x * q, n[2].y - kz * kd.y * q};
+  } else {
+    n[0] = {n[0].x - kx * kd.x / k2s, n[0].y - kx * kd.y / k2s};
+    n[1] = {n[1].x - ky * kd.x / k2s, n[1].y - ky * kd.y / k2s};
+    n[2] = {n[2].x - kz * kd.x / k2s, n[2].y - kz * kd.y / k2s};
+  }
 }
 
 // RK4 bookkeeping shared by the u and b halves of the state.
