@@ -24,7 +24,7 @@ if ROOT not in sys.path:
 
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hydro_256_fp64_traffic.txt")
-KERNEL_OF_PASS = {"inv_x": "k_inv_x<double, 256, 0, {v}>", "inv_y": "k_y<double, 256, 1, {v}>",
+KERNEL_OF_PASS = {"inv_x": "k_inv_x<double, 256, 7, {v}>", "inv_y": "k_y_curl<double, 256, {v}>",
                   "z": ["k_zw<double, 256, 16, 4>", "k_zw<double, 256, 8, 4>", "k_z<double, 256, 0, 2>"],
                   "fwd_y": "k_y<double, 256, -1, {v}>",
                   "fwd_x": ["k_fwd_x<double, 256, 0, 0>", "k_fwd_xr<double, 256, false>", "k_fwd_x<double, 256, 0, 2>"]}

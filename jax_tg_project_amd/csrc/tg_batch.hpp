@@ -63,11 +63,13 @@ __device__ __forceinline__ void ens_run(cpx<R>* sm, int lb, const EnsInst<R>& I,
   using Gd = EnsGrid<R, N>;
   if constexpr (PASS == EP_INV_X) {
     const cpx<R>* us = stage == 0 ? I.u0 : I.u1;
-    XInvBody<R, N, XM_HYDRO, 0>::run(sm, lb % Gd::GX_XY, lb / Gd::GX_XY, us, nullptr, I.A, I.tw, I.ks);
-  } else if constexpr (PASS == EP_INV_Y || PASS == EP_FWD_Y) {
+    XInvBody<R, N, XM_HYDRO5, 0>::run(sm, lb % Gd::GX_XY, lb / Gd::GX_XY, us, nullptr, I.A, I.tw, I.ks);
+  } else if constexpr (PASS == EP_INV_Y) {
     const int bx = lb % Gd::GX_XY, rest = lb / Gd::GX_XY;
-    if constexpr (PASS == EP_INV_Y) YBody<R, N, +1, 0>::run(sm, bx, rest % N, rest / N, I.A, I.B, I.tw);
-    else YBody<R, N, -1, 0>::run(sm, bx, rest % N, rest / N, I.A, I.B, I.tw);
+    YCurlBody<R, N, 0>::run(sm, bx, rest % N, rest / N, 0, I.A, I.B, I.tw, I.ks);
+  } else if constexpr (PASS == EP_FWD_Y) {
+    const int bx = lb % Gd::GX_XY, rest = lb / Gd::GX_XY;
+    YBody<R, N, -1, 0>::run(sm, bx, rest % N, rest / N, I.A, I.B, I.tw);
   } else if constexpr (PASS == EP_Z) {
     if constexpr (Gd::ZE > 0) {
       ZWBody<R, N, Gd::ZE, Gd::ZWPB>::run(sm, lb, I.B, I.A, I.tw);
@@ -106,7 +108,8 @@ template <typename R, int N, int PASS>
 struct EnsBlock {
   static constexpr int NT = PASS == EP_Z ? EnsZ<R, N>::NT
                           : PASS == EP_FWD_X ? CfgXF<R, N, 0>::NT : CfgXY<R, N, 0>::NT;
-  static constexpr int SMEM = PASS == EP_INV_X ? XInvBody<R, N, XM_HYDRO, 0>::SMEM
+  static constexpr int SMEM = PASS == EP_INV_X ? XInvBody<R, N, XM_HYDRO5, 0>::SMEM
+                            : PASS == EP_INV_Y ? YCurlBody<R, N, 0>::SMEM
                             : PASS == EP_Z ? EnsZ<R, N>::SMEM
                             : PASS == EP_FWD_X ? XFBody<R, N, FM_HYDRO_STAGE, 0>::SMEM
                             : YBody<R, N, 1, 0>::SMEM;

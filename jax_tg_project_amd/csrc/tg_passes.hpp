@@ -82,24 +82,42 @@ struct CfgXF {
   static constexpr int NT = LINES * N / E;
 };
 
-enum XMode { XM_HYDRO = 0, XM_MHD = 1, XM_COPY3 = 2, XM_DIV = 3, XM_CURL3 = 4, XM_ADJ = 5, XM_ADJQ = 6 };
+enum XMode { XM_HYDRO = 0, XM_MHD = 1, XM_COPY3 = 2, XM_DIV = 3, XM_CURL3 = 4, XM_ADJ = 5, XM_ADJQ = 6,
+             XM_HYDRO5 = 7, XM_MHD5 = 8 };
 enum ZMode { ZM_HYDRO = 0, ZM_MHD = 1, ZM_PHYS_OUT3 = 2, ZM_MAXABS = 3, ZM_PHYS_IN3 = 4, ZM_PHYS_OUT1 = 5, ZM_ADJ = 6 };
 enum FMode { FM_HYDRO_STAGE = 0, FM_MHD_STAGE = 1, FM_STORE3 = 2, FM_HYDRO_RHS = 3, FM_MHD_RHS = 4, FM_ADJ = 5 };
 
 __host__ __device__ constexpr int xmode_fields(int m) {
-  return m == XM_HYDRO ? 6 : m == XM_MHD ? 12 : m == XM_ADJ ? 9 : (m == XM_COPY3 || m == XM_CURL3 || m == XM_ADJQ) ? 3 : 1;
+  return m == XM_HYDRO ? 6 : m == XM_MHD ? 12 : m == XM_ADJ ? 9 : m == XM_HYDRO5 ? 5 : m == XM_MHD5 ? 10
+       : (m == XM_COPY3 || m == XM_CURL3 || m == XM_ADJQ) ? 3 : 1;
 }
 
 // ----------------------------------------------------------------- pass 1
 // Inverse transform along x of the fields derived from the compact state:
 // HYDRO -> (u, omega) = (S, i k x S); MHD -> (u, omega, b, j); COPY3 -> S;
 // CURL3 -> i k x S; DIV -> i k . S; ADJ -> (u, omega, P(alpha a)) of the
-// state and the cotangent; ADJQ -> P(alpha S) of a cotangent S alone.  The
-// 1/N^3 inverse normalisation is applied here.
+// state and the cotangent; ADJQ -> P(alpha S) of a cotangent S alone;
+// HYDRO5 -> (u, i kx u_y, i kx u_z), the five fields from which the y pass
+// completes omega (k_y_curl); MHD5 -> the same for u and for b.  The 1/N^3
+// inverse normalisation is applied here.
+template <typename R>
+__device__ __forceinline__ cpx<R> xfield5(int f, cpx<R> a, cpx<R> b, cpx<R> c, R kx) {
+  if (f == 0) return a;
+  if (f == 1) return b;
+  if (f == 2) return c;
+  return cmuli(f == 3 ? cpx<R>{kx * b.x, kx * b.y} : cpx<R>{kx * c.x, kx * c.y});
+}
+
 template <typename R, int N, int MODE>
 __device__ __forceinline__ cpx<R> xfield(int f, const cpx<R>* __restrict__ su,
                                          const cpx<R>* __restrict__ sb, size_t pt, size_t cs,
                                          R kx, R ky, R kz, R alpha = R(1)) {
+  if constexpr (MODE == XM_HYDRO5 || MODE == XM_MHD5) {
+    const cpx<R>* s = f >= 5 ? sb : su;
+    const int g = f >= 5 ? f - 5 : f;
+    if (g < 3) return s[g * cs + pt];
+    return xfield5<R>(g, cpx<R>{}, s[cs + pt], s[2 * cs + pt], kx);
+  }
   if constexpr (MODE == XM_ADJQ) {  // P(alpha * a) of the cotangent su
     sb = su;
     f += 6;
@@ -141,6 +159,10 @@ template <typename R, int N, int MODE, int NS, int E>
 __device__ __forceinline__ cpx<R> xfield_reg(int f, const cpx<R> (&sv)[NS][E], int slot, R kx, R ky, R kz,
                                              R alpha) {
   int o = 0;
+  if constexpr (MODE == XM_HYDRO5 || MODE == XM_MHD5) {
+    if (f >= 5) { o = 3; f -= 5; }
+    return xfield5<R>(f, sv[o][slot], sv[o + 1][slot], sv[o + 2][slot], kx);
+  }
   if constexpr (MODE == XM_ADJ || MODE == XM_ADJQ) {
     if (MODE == XM_ADJQ || f >= 6) {
       constexpr int q0 = MODE == XM_ADJQ ? 0 : 3;  // components of the cotangent
@@ -195,7 +217,7 @@ struct XInvBody {
     // The first FFT stage of every field reads the same E points per thread:
     // load the state (and cotangent) components at those points once.
     constexpr int NS = (MODE == XM_HYDRO || MODE == XM_CURL3 || MODE == XM_DIV || MODE == XM_COPY3) ? 3
-                     : (MODE == XM_MHD || MODE == XM_ADJ) ? 6 : 3;
+                     : (MODE == XM_MHD || MODE == XM_MHD5 || MODE == XM_ADJ) ? 6 : 3;
     constexpr int TPL = N / E;
     const int tid = threadIdx.x;
     const int l = tid % L, t = tid / L;
@@ -323,6 +345,84 @@ __global__ __launch_bounds__((CfgXY<R, N, V>::NT)) void k_y(const cpx<R>* __rest
   using B = YBody<R, N, DIR, V>;
   __shared__ cpx<R> sm[B::SMEM];
   B::run(sm, blockIdx.x, blockIdx.y, blockIdx.z, in, out, tw);
+}
+
+// Inverse y transform that completes the curl.  The x pass (HYDRO5 / MHD5)
+// hands over five fields per vector v: (v_x, v_y, v_z, D_y, D_z) with
+// D_c = i kx v_c after the x transform.  While ky and kz are still spectral
+// the remaining derivatives are one multiply each, so this pass writes the
+// six fields (v, i k x v):
+//   w_x = i (ky v_z - kz v_y),   w_y = i kz v_x - D_z,   w_z = D_y - i ky v_x.
+// Compared with transforming six fields along x, the x pass writes and this
+// pass reads one field in six less.  One block transforms one output field
+// of one (x, kz tile); the components of v are read by up to three of the
+// six blocks of that tile, with ordinary loads so that the copies the other
+// blocks fetched serve them from L2 (the launcher places the six blocks on
+// one XCD, one after another), and D_y, D_z once, streamed.
+template <typename R, int N, int V>
+struct YCurlBody {
+  using G = Geo<N>;
+  using C = CfgXY<R, N, V>;
+  static constexpr int SMEM = N * C::L + N;
+  static constexpr size_t AF = size_t(N) * G::K * kzp<R, N>();  // field stride of A
+  static constexpr size_t BF = size_t(N) * N * kzp<R, N>();     // field stride of B
+  // (bx, x, f, vec): kz tile, x plane, output field 0-5, vector.  A[5 vec ..]
+  // -> B[6 vec + f].
+  __device__ __forceinline__ static void run(cpx<R>* sm, int bx, int x, int f, int vec,
+                                             const cpx<R>* __restrict__ in, cpx<R>* __restrict__ out,
+                                             const cpx<R>* __restrict__ tw, R ks) {
+    constexpr int L = C::L, E = C::E, KZP = kzp<R, N>();
+    cpx<R>* buf = sm;
+    cpx<R>* tws = sm + N * L;
+    const int iz0 = bx * L;
+    const cpx<R>* a = in + size_t(5 * vec) * AF + size_t(x) * G::K * KZP;
+    cpx<R>* b = out + (size_t(6 * vec) + f) * BF + size_t(x) * N * KZP;
+    stage_twiddles<R, N>(tws, tw);
+    auto ld = [&](int n, int ll, int) -> cpx<R> {
+      const int izr = iz0 + ll;
+      const bool colok = izr < G::KZ;
+      const bool ok = colok && G::kept(n);
+      const int iy = ok ? G::compact_of(n) : 0;
+      const size_t pt = size_t(iy) * KZP + (colok ? izr : G::KZ - 1);
+      const R ky = ks * R(G::kint(iy)), kz = ks * R(izr);
+      cpx<R> v;
+      if (f < 3) {
+        v = gld<false>(&a[f * AF + pt]);
+      } else if (f == 3) {
+        const cpx<R> y = gld<false>(&a[AF + pt]), z = gld<false>(&a[2 * AF + pt]);
+        v = cmuli(cpx<R>{ky * z.x - kz * y.x, ky * z.y - kz * y.y});
+      } else {
+        const cpx<R> vx = gld<false>(&a[pt]);
+        const cpx<R> d = gld<nt_mid<N>()>(&a[(f == 4 ? 4 : 3) * AF + pt]);
+        const R k = f == 4 ? kz : ky;
+        const cpx<R> t = cmuli(cpx<R>{k * vx.x, k * vx.y});
+        v = f == 4 ? cpx<R>{t.x - d.x, t.y - d.y} : cpx<R>{d.x - t.x, d.y - t.y};
+      }
+      return ok ? v : cpx<R>{R(0), R(0)};
+    };
+    auto st = [&](int n, int ll, cpx<R> v, int) {
+      const int iz = iz0 + ll;
+      if (iz < G::KZ) gst<nt_mid<N>()>(&b[size_t(n) * KZP + iz], v);
+    };
+    fft_io<R, N, E, +1, 1, true>(buf, L, L, tws, ld, st);
+  }
+};
+
+// Linear block b of a 1-D grid of 48 * ceil(U / 8) blocks, U = tiles x N x
+// vectors: workgroups go to the 8 XCDs round robin (b mod 8), so the six
+// fields of tile u take the consecutive slots 6 j .. 6 j + 5 of XCD u mod 8
+// (j = u / 8).  Blocks past the last tile return at once.
+template <typename R, int N, int V>
+__global__ __launch_bounds__((CfgXY<R, N, V>::NT)) void k_y_curl(const cpx<R>* __restrict__ in,
+                                                               cpx<R>* __restrict__ out,
+                                                               const cpx<R>* __restrict__ tw, R ks, int units) {
+  using B = YCurlBody<R, N, V>;
+  constexpr int T = (Geo<N>::KZ + CfgXY<R, N, V>::L - 1) / CfgXY<R, N, V>::L;
+  __shared__ cpx<R> sm[B::SMEM];
+  const int b = blockIdx.x, xcd = b & 7, j = b >> 3;
+  const int u = (j / 6) * 8 + xcd;
+  if (u >= units) return;
+  B::run(sm, u % T, (u / T) % N, j % 6, u / (T * N), in, out, tw, ks);
 }
 
 // ----------------------------------------------------------------- pass 3

@@ -140,6 +140,8 @@ struct Launch {
         case XM_CURL3: hipLaunchKernelGGL((k_inv_x<R, N, XM_CURL3, V>), grid, blk, 0, s, su, sb, out, tw, ks, al, cpz); break;
         case XM_ADJ: hipLaunchKernelGGL((k_inv_x<R, N, XM_ADJ, V>), grid, blk, 0, s, su, sb, out, tw, ks, al, cpz); break;
         case XM_ADJQ: hipLaunchKernelGGL((k_inv_x<R, N, XM_ADJQ, V>), grid, blk, 0, s, su, sb, out, tw, ks, al, cpz); break;
+        case XM_HYDRO5: hipLaunchKernelGGL((k_inv_x<R, N, XM_HYDRO5, V>), grid, blk, 0, s, su, sb, out, tw, ks, al, cpz); break;
+        case XM_MHD5: hipLaunchKernelGGL((k_inv_x<R, N, XM_MHD5, V>), grid, blk, 0, s, su, sb, out, tw, ks, al, cpz); break;
         default: hipLaunchKernelGGL((k_inv_x<R, N, XM_DIV, V>), grid, blk, 0, s, su, sb, out, tw, ks, al, cpz); break;
       }
       TG_LAUNCHED();
@@ -153,6 +155,18 @@ struct Launch {
       dim3 grid(cdiv(G::KZ, C::L), N, nf), blk(C::NT);
       if (dir > 0) hipLaunchKernelGGL((k_y<R, N, +1, V>), grid, blk, 0, s, in, out, tw);
       else hipLaunchKernelGGL((k_y<R, N, -1, V>), grid, blk, 0, s, in, out, tw);
+      TG_LAUNCHED();
+      return TG_OK;
+    });
+  }
+  // Inverse y of `nvec` vectors handed over as five fields each (HYDRO5 /
+  // MHD5 x pass), writing (v, i k x v) per vector.
+  static int y_curl(int var, int nvec, const cpx<R>* in, cpx<R>* out, const cpx<R>* tw, R ks, hipStream_t s) {
+    return with_variant<N>(var, [&](auto vc) {
+      constexpr int V = decltype(vc)::value;
+      using C = CfgXY<R, N, V>;
+      const int units = cdiv(G::KZ, C::L) * N * nvec;
+      hipLaunchKernelGGL((k_y_curl<R, N, V>), dim3(48 * cdiv(units, 8)), dim3(C::NT), 0, s, in, out, tw, ks, units);
       TG_LAUNCHED();
       return TG_OK;
     });
@@ -325,12 +339,12 @@ int rhs_passes(const tg_plan* p, const Ptrs<R>& w, const cpx<R>* us, const cpx<R
   using Lc = Launch<R, N>;
   const cpx<R>* tw = static_cast<const cpx<R>*>(p->tw);
   const bool mhd = p->mhd != 0;
-  const int fi = mhd ? 12 : 6, fo = mhd ? 6 : 3;
+  const int fo = mhd ? 6 : 3;
   const int* v = p->var;
-  TG_TIMED(TG_PASS_INV_X, Lc::inv_x(mhd ? XM_MHD : XM_HYDRO, v[TG_PASS_INV_X], us, bs, w.A, tw, R(p->ks), s, R(1),
+  TG_TIMED(TG_PASS_INV_X, Lc::inv_x(mhd ? XM_MHD5 : XM_HYDRO5, v[TG_PASS_INV_X], us, bs, w.A, tw, R(p->ks), s, R(1),
                                     cpz));
   cpx<R>* bm = bslot ? bslot : w.B;
-  TG_TIMED(TG_PASS_INV_Y, Lc::y(+1, v[TG_PASS_INV_Y], fi, w.A, bm, tw, s));
+  TG_TIMED(TG_PASS_INV_Y, Lc::y_curl(v[TG_PASS_INV_Y], mhd ? 2 : 1, w.A, bm, tw, R(p->ks), s));
   TG_TIMED(TG_PASS_Z, Lc::z(mhd ? ZM_MHD : ZM_HYDRO, v[TG_PASS_Z], bm, w.A, nullptr, nullptr, tw, lor, s));
   TG_TIMED(TG_PASS_FWD_Y, Lc::y(-1, v[TG_PASS_FWD_Y], fo, w.A, w.B, tw, s));
   TG_TIMED(TG_PASS_FWD_X, Lc::fwd_x(fmode, v[TG_PASS_FWD_X], w.B, a, tw, s));

@@ -8,9 +8,12 @@ that pass), with c = bytes per complex element (16 FP64, 8 FP32):
     K K KZ  compact state components
 
 pass     reads                      writes
-inv_x    3 K K KZ  (6 MHD)          F N K KZ     (F = 6 hydro, 12 MHD)
-inv_y    F N K KZ                   F N N KZ
+inv_x    3 K K KZ  (6 MHD)          X N K KZ     (X = 5 hydro, 10 MHD)
+inv_y    X N K KZ                   F N N KZ     (F = 6 hydro, 12 MHD)
 z        F N N KZ                   G N N KZ     (G = 3 hydro, 6 MHD)
+
+The x pass hands over (v, i kx v_y, i kx v_z) per vector v and the y pass
+completes i k x v, so X = 5 fields per vector cross the x/y boundary.
 fwd_y    G N N KZ                   G N K KZ
 fwd_x    G N K KZ + stage vectors   stage vectors
 
@@ -31,13 +34,13 @@ def pass_bytes(N, prec_bytes=8, mhd=False):
     """Bytes per launch for each pass name; fwd_x is a list over the 4 stages."""
     _, K, KZ = sizes(N)
     c = 2 * prec_bytes
-    F, G = (12, 6) if mhd else (6, 3)
+    X, F, G = (10, 12, 6) if mhd else (5, 6, 3)
     nkk = N * K * KZ * c
     nnk = N * N * KZ * c
     vec = 3 * K * K * KZ * c * (MHD_FACTOR if mhd else 1)
     return {
-        "inv_x": vec + F * nkk,
-        "inv_y": F * nkk + F * nnk,
+        "inv_x": vec + X * nkk,
+        "inv_y": X * nkk + F * nnk,
         "z": F * nnk + G * nnk,
         "fwd_y": G * nnk + G * nkk,
         "fwd_x": [G * nkk + v * vec for v in (3, 5, 5, 4)],

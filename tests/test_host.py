@@ -155,8 +155,12 @@ def test_roofline_byte_model():
     pb = roofline.pass_bytes(256, 8)
     # z pass: reads 6 and writes 3 fields of N*N*KZ complex doubles
     assert pb["z"] == 9 * 256 * 256 * 86 * 16
+    # x pass: reads 3 components K*K*KZ, writes (u, i kx u_y, i kx u_z) over N*K*KZ;
+    # y pass: reads those 5 fields and writes (u, curl u) over N*N*KZ
+    assert pb["inv_x"] == (3 * 171 * 171 + 5 * 256 * 171) * 86 * 16
+    assert pb["inv_y"] == (5 * 256 * 171 + 6 * 256 * 256) * 86 * 16
     total = roofline.step_bytes(256, 8)
-    assert 13.0e9 < total < 13.8e9
+    assert 12.6e9 < total < 13.2e9
     assert roofline.step_bytes(256, 4) * 2 == total
     assert roofline.step_bytes(128, 8, mhd=True) > 1.9 * roofline.step_bytes(128, 8)
 
