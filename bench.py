@@ -189,7 +189,9 @@ def run_adjoint(N, steps, dt, nu, dtype):
     from jax_tg_project_amd._lib import Plan
     from jax_tg_project_amd import ensemble
     from jax_tg_project_amd.adjoint import sqrt_segment
+    from jax_tg_project_amd import autotune
     plan = Plan(N, dtype=dtype)
+    tuned = autotune.tune(plan, use_cache=False)[0]  # forward and reverse sweeps run the tuned tiles
     x0 = plan.forward(torch.tensor(ensemble.perturbed_taylor_green(N, 2024), dtype=dtype, device=plan.device))
     seg = sqrt_segment(steps)
     y = x0.clone()
@@ -201,7 +203,8 @@ def run_adjoint(N, steps, dt, nu, dtype):
     plan.rk4(y, nsteps=steps, dt=dt, nu=nu)
     torch.cuda.synchronize()
     fwd = time.perf_counter() - t0
-    out = {"N": N, "steps": steps, "dtype": "f64" if dtype == torch.float64 else "f32", "forward_s": fwd}
+    out = {"N": N, "steps": steps, "dtype": "f64" if dtype == torch.float64 else "f32", "tile_variants": tuned,
+           "forward_s": fwd}
     for name, sg in (("sqrt_n", seg), ("all_stages_kept", steps)):
         aws = plan.adjoint_workspace(steps, sg)
         torch.cuda.synchronize()

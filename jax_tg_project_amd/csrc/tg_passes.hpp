@@ -1133,6 +1133,46 @@ struct StageArgs {
   double* nupart;
 };
 
+// Adjoint stage bookkeeping of one compact point (the FM_ADJ contract of
+// StageArgs): v[0..2] is the forward-x output of the product's VJP through
+// u, v[3..5] the field whose curl adds to it.  Every load of the point is
+// issued before its stores; returns the point's term of the nu gradient.
+template <typename R>
+__device__ __forceinline__ double adj_point(const StageArgs<R>& a, size_t pt, size_t cs, R kx, R ky, R kz,
+                                            int iz, const cpx<R> (&v)[6]) {
+  cpx<R> ar[3], lp[3], l0[3], xv[3];
+  const bool uselp = a.first == 1 || !a.last;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) ar[c] = a.a[c * cs + pt];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) lp[c] = uselp ? a.lamp[c * cs + pt] : cpx<R>{R(0), R(0)};
+#pragma unroll
+  for (int c = 0; c < 3; ++c) l0[c] = a.first == 0 ? a.lacc[c * cs + pt] : cpx<R>{R(0), R(0)};
+#pragma unroll
+  for (int c = 0; c < 3; ++c) xv[c] = a.xs ? a.xs[c * cs + pt] : cpx<R>{R(0), R(0)};
+  const R k2 = kx * kx + ky * ky + kz * kz;
+  const R nk = a.nu * k2;
+  cpx<R> cb[3];
+  cb[0] = cmuli(cpx<R>{ky * v[5].x - kz * v[4].x, ky * v[5].y - kz * v[4].y});
+  cb[1] = cmuli(cpx<R>{kz * v[3].x - kx * v[5].x, kz * v[3].y - kx * v[5].y});
+  cb[2] = cmuli(cpx<R>{kx * v[4].x - ky * v[3].x, kx * v[4].y - ky * v[3].y});
+  const double wk = (iz == 0 ? 1.0 : 2.0) * double(k2);
+  double nus = 0.0;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const size_t o = c * cs + pt;
+    const cpx<R> as = {a.alpha * ar[c].x, a.alpha * ar[c].y};
+    const cpx<R> mu = {v[c].x + cb[c].x - nk * as.x, v[c].y + cb[c].y - nk * as.y};
+    cpx<R> acc = mu;
+    if (a.first == 1) acc = {lp[c].x + mu.x, lp[c].y + mu.y};
+    else if (a.first == 0) acc = {l0[c].x + mu.x, l0[c].y + mu.y};
+    if (a.xs) nus += wk * (double(as.x) * double(xv[c].x) + double(as.y) * double(xv[c].y));
+    a.lacc[o] = acc;
+    if (!a.last) a.anext[o] = {a.clam * lp[c].x + a.cmu * mu.x, a.clam * lp[c].y + a.cmu * mu.y};
+  }
+  return nus;
+}
+
 // Leray projection n - k (k.n) / k2 (k2s: k2 with 0 -> 1).  FP64 divides
 // each k_c (k.n) by k2 as the reference writes it; FP32, whose passes are
 // bound by instruction issue, multiplies by one reciprocal instead (a last-
@@ -1152,29 +1192,54 @@ __device__ __forceinline__ void project(R kx, R ky, R kz, R k2s, cpx<R>* n) {
   }
 }
 
-// RK4 bookkeeping shared by the u and b halves of the state.
+// RK4 bookkeeping shared by the u and b halves of the state, split into the
+// loads of a point (x0, and the accumulator after stage 0) and the update
+// and stores: a caller issues every load of its point before any store, so
+// the point costs one memory round trip (loads that follow a store in
+// program order also wait for that store).
 template <typename R>
-__device__ __forceinline__ void rk4_update(int stage, R dt, size_t pt, size_t cs, const cpx<R>* r,
-                                           const cpx<R>* x0, cpx<R>* xnext, cpx<R>* acc) {
+struct RK4In {
+  cpx<R> x0[3], acc[3];
+};
+
+template <typename R>
+__device__ __forceinline__ RK4In<R> rk4_load(int stage, size_t pt, size_t cs, const cpx<R>* x0, const cpx<R>* acc) {
+  RK4In<R> in;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) in.x0[c] = x0[c * cs + pt];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) in.acc[c] = stage > 0 ? acc[c * cs + pt] : cpx<R>{R(0), R(0)};
+  return in;
+}
+
+template <typename R>
+__device__ __forceinline__ void rk4_store(int stage, R dt, size_t pt, size_t cs, const cpx<R>* r, const RK4In<R>& in,
+                                          cpx<R>* xnext, cpx<R>* acc) {
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     const size_t o = c * cs + pt;
-    const cpx<R> a0 = x0[o];
+    const cpx<R> a0 = in.x0[c];
     if (stage == 0) {
       acc[o] = r[c];
       const R h = R(0.5) * dt;
       xnext[o] = {a0.x + h * r[c].x, a0.y + h * r[c].y};
     } else if (stage < 3) {
-      const cpx<R> s = acc[o];
+      const cpx<R> s = in.acc[c];
       acc[o] = {s.x + R(2) * r[c].x, s.y + R(2) * r[c].y};
       const R h = (stage == 1) ? R(0.5) * dt : dt;
       xnext[o] = {a0.x + h * r[c].x, a0.y + h * r[c].y};
     } else {
-      const cpx<R> s = acc[o];
+      const cpx<R> s = in.acc[c];
       const R h = dt / R(6);
       xnext[o] = {a0.x + h * (s.x + r[c].x), a0.y + h * (s.y + r[c].y)};
     }
   }
+}
+
+template <typename R>
+__device__ __forceinline__ void rk4_update(int stage, R dt, size_t pt, size_t cs, const cpx<R>* r,
+                                           const cpx<R>* x0, cpx<R>* xnext, cpx<R>* acc) {
+  rk4_store(stage, dt, pt, cs, r, rk4_load(stage, pt, cs, x0, acc), xnext, acc);
 }
 
 template <typename R, int N, int MODE, int V>
@@ -1219,44 +1284,29 @@ struct XFBody {
       continue;
     } else if constexpr (MODE == FM_ADJ) {
       const R kx = a.ks * R(G::kint(ix)), kz = a.ks * R(iz);
-      const R k2 = kx * kx + ky * ky + kz * kz;
-      const R nk = a.nu * k2;
-      cpx<R> cb[3];
-      cb[0] = cmuli(cpx<R>{ky * v[5].x - kz * v[4].x, ky * v[5].y - kz * v[4].y});
-      cb[1] = cmuli(cpx<R>{kz * v[3].x - kx * v[5].x, kz * v[3].y - kx * v[5].y});
-      cb[2] = cmuli(cpx<R>{kx * v[4].x - ky * v[3].x, kx * v[4].y - ky * v[3].y});
-      const double wk = (iz == 0 ? 1.0 : 2.0) * double(k2);
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        const size_t o = c * cs + pt;
-        const cpx<R> ar = a.a[o];
-        const cpx<R> as = {a.alpha * ar.x, a.alpha * ar.y};
-        const cpx<R> mu = {v[c].x + cb[c].x - nk * as.x, v[c].y + cb[c].y - nk * as.y};
-        const cpx<R> lp = (a.first == 1 || !a.last) ? a.lamp[o] : cpx<R>{R(0), R(0)};
-        cpx<R> acc = mu;
-        if (a.first == 1) acc = {lp.x + mu.x, lp.y + mu.y};
-        else if (a.first == 0) { const cpx<R> l0 = a.lacc[o]; acc = {l0.x + mu.x, l0.y + mu.y}; }
-        if (a.xs) {
-          const cpx<R> xv = a.xs[o];
-          nusum += wk * (double(as.x) * double(xv.x) + double(as.y) * double(xv.y));
-        }
-        a.lacc[o] = acc;
-        if (!a.last) a.anext[o] = {a.clam * lp.x + a.cmu * mu.x, a.clam * lp.y + a.cmu * mu.y};
-      }
+      nusum += adj_point(a, pt, cs, kx, ky, kz, iz, v);
       continue;
     } else {
       const R kx = a.ks * R(G::kint(ix)), kz = a.ks * R(iz);
       const R k2 = kx * kx + ky * ky + kz * kz;
       const R k2s = (k2 == R(0)) ? R(1) : k2;
+      // every load of the point first (see rk4_load)
+      cpx<R> su[3], sbv[3];
+      RK4In<R> inu{}, inb{};
+#pragma unroll
+      for (int c = 0; c < 3; ++c) su[c] = a.us[c * cs + pt];
+      if constexpr (MODE == FM_HYDRO_STAGE || MODE == FM_MHD_STAGE) inu = rk4_load(a.stage, pt, cs, a.u0, a.uacc);
+      if constexpr (FF == 6) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) sbv[c] = a.bs[c * cs + pt];
+        if constexpr (MODE == FM_MHD_STAGE) inb = rk4_load(a.stage, pt, cs, a.b0, a.bacc);
+      }
       // momentum: r_u = P(u x w [+ lorentz j x b]) - nu k2 u_s
       project(kx, ky, kz, k2s, v);
       cpx<R> ru[3];
       const R nk = a.nu * k2;
 #pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        const cpx<R> s = a.us[c * cs + pt];
-        ru[c] = {v[c].x - nk * s.x, v[c].y - nk * s.y};
-      }
+      for (int c = 0; c < 3; ++c) ru[c] = {v[c].x - nk * su[c].x, v[c].y - nk * su[c].y};
       if constexpr (FF == 6) {
         // induction: r_b = P(i k x e) - eta k2 b_s
         const cpx<R> e0 = v[3], e1 = v[4], e2 = v[5];
@@ -1268,23 +1318,20 @@ struct XFBody {
         const R ek = a.eta * k2;
         cpx<R> rb[3];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          const cpx<R> s = a.bs[c * cs + pt];
-          rb[c] = {cb[c].x - ek * s.x, cb[c].y - ek * s.y};
-        }
+        for (int c = 0; c < 3; ++c) rb[c] = {cb[c].x - ek * sbv[c].x, cb[c].y - ek * sbv[c].y};
         if constexpr (MODE == FM_MHD_RHS) {
 #pragma unroll
           for (int c = 0; c < 3; ++c) { a.unext[c * cs + pt] = ru[c]; a.bnext[c * cs + pt] = rb[c]; }
         } else {
-          rk4_update(a.stage, a.dt, pt, cs, ru, a.u0, a.unext, a.uacc);
-          rk4_update(a.stage, a.dt, pt, cs, rb, a.b0, a.bnext, a.bacc);
+          rk4_store(a.stage, a.dt, pt, cs, ru, inu, a.unext, a.uacc);
+          rk4_store(a.stage, a.dt, pt, cs, rb, inb, a.bnext, a.bacc);
         }
       } else {
         if constexpr (MODE == FM_HYDRO_RHS) {
 #pragma unroll
           for (int c = 0; c < 3; ++c) a.unext[c * cs + pt] = ru[c];
         } else {
-          rk4_update(a.stage, a.dt, pt, cs, ru, a.u0, a.unext, a.uacc);
+          rk4_store(a.stage, a.dt, pt, cs, ru, inu, a.unext, a.uacc);
         }
       }
     }
@@ -1328,12 +1375,17 @@ struct CfgXR {
   static constexpr int NT = L * N / E;
 };
 
-template <typename R, int N, bool MHD = false>
+// Epilogues of the register forward-x pass: hydro or MHD RK4 stage, or the
+// adjoint bookkeeping of one stage VJP (the FM_ADJ epilogue of k_fwd_x).
+enum XRKind { XR_HYDRO = 0, XR_MHD = 1, XR_ADJ = 2 };
+
+template <typename R, int N, int KIND = XR_HYDRO>
 struct XRBody {
   using G = Geo<N>;
   using C = CfgXR<R, N>;
+  static constexpr bool MHD = KIND == XR_MHD;
   static constexpr int SMEM = N * C::L + N;
-  static constexpr int FF = MHD ? 6 : 3;  // fields: n (hydro); m, e (MHD)
+  static constexpr int FF = KIND == XR_HYDRO ? 3 : 6;  // fields: n (hydro); m, e (MHD); adjoint 6
   __device__ __forceinline__ static void run(cpx<R>* sm, int bx, int by, const cpx<R>* __restrict__ in,
                                              const StageArgs<R>& a, const cpx<R>* __restrict__ tw) {
     constexpr int L = C::L, E = C::E;
@@ -1358,6 +1410,10 @@ struct XRBody {
       auto st = [&](int, int, cpx<R> v, int slot) { o[f][slot] = v; };
       fft_io<R, N, E, -1, 1, true>(buf, L, L, tws, ld, st);
     }
+    if constexpr (KIND == XR_ADJ) {
+      adjoint_epilogue(buf, o, colok, t, iy, iz, a);
+      return;
+    }
     if (!colok) return;
     const int cpz = a.cpz ? a.cpz : G::KZ;
     const size_t cs = size_t(G::K) * G::K * cpz;
@@ -1375,18 +1431,24 @@ struct XRBody {
         const R kx = a.ks * R(G::kint(ix));
         const R k2 = kx * kx + ky * ky + kz * kz;
         const R k2s = (k2 == R(0)) ? R(1) : k2;
+        // every load of the u half before its stores (see rk4_load)
+        cpx<R> su[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) su[c] = a.us[c * cs + pt];
+        const RK4In<R> inu = rk4_load(a.stage, pt, cs, a.u0, a.uacc);
         // momentum: r_u = P(n or m) - nu k2 u_s
         project(kx, ky, kz, k2s, v);
         cpx<R> ru[3];
         const R nk = a.nu * k2;
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          const cpx<R> s = a.us[c * cs + pt];
-          ru[c] = {v[c].x - nk * s.x, v[c].y - nk * s.y};
-        }
-        rk4_update(a.stage, a.dt, pt, cs, ru, a.u0, a.unext, a.uacc);
+        for (int c = 0; c < 3; ++c) ru[c] = {v[c].x - nk * su[c].x, v[c].y - nk * su[c].y};
+        rk4_store(a.stage, a.dt, pt, cs, ru, inu, a.unext, a.uacc);
         if constexpr (MHD) {
-          // induction: r_b = P(i k x e) - eta k2 b_s
+          // induction: r_b = P(i k x e) - eta k2 b_s (the b half's loads, then its stores)
+          cpx<R> sbv[3];
+#pragma unroll
+          for (int c = 0; c < 3; ++c) sbv[c] = a.bs[c * cs + pt];
+          const RK4In<R> inb = rk4_load(a.stage, pt, cs, a.b0, a.bacc);
           const cpx<R> e0 = o[3][j * RL + m], e1 = o[4][j * RL + m], e2 = o[5][j * RL + m];
           cpx<R> cb[3];
           cb[0] = cmuli(cpx<R>{ky * e2.x - kz * e1.x, ky * e2.y - kz * e1.y});
@@ -1396,21 +1458,80 @@ struct XRBody {
           const R ek = a.eta * k2;
           cpx<R> rb[3];
 #pragma unroll
-          for (int c = 0; c < 3; ++c) {
-            const cpx<R> s = a.bs[c * cs + pt];
-            rb[c] = {cb[c].x - ek * s.x, cb[c].y - ek * s.y};
-          }
-          rk4_update(a.stage, a.dt, pt, cs, rb, a.b0, a.bnext, a.bacc);
+          for (int c = 0; c < 3; ++c) rb[c] = {cb[c].x - ek * sbv[c].x, cb[c].y - ek * sbv[c].y};
+          rk4_store(a.stage, a.dt, pt, cs, rb, inb, a.bnext, a.bacc);
         }
+      }
+    }
+  }
+
+  // Stage-VJP bookkeeping on the register-held lines (o[0..2]: the VJP of
+  // the product through u, o[3..5]: the field whose curl adds to it), as
+  // the FM_ADJ epilogue of XFBody states it; the block's share of the nu
+  // gradient goes to a.nupart[block].  Every thread reaches the barriers.
+  __device__ __forceinline__ static void adjoint_epilogue(cpx<R>* buf, const cpx<R> (&o)[FF][C::E], bool colok,
+                                                          int t, int iy, int iz, const StageArgs<R>& a) {
+    constexpr int E = C::E;
+    using Pl = typename PlanFor<N, E>::type;
+    constexpr int RL = LastRadix<Pl>::value, NBL = E / RL, SL = N / RL;
+    double nusum = 0.0;
+    if (colok) {
+      const int cpz = a.cpz ? a.cpz : G::KZ;
+      const size_t cs = size_t(G::K) * G::K * cpz;
+      const R ky = a.ks * R(G::kint(iy)), kz = a.ks * R(iz);
+#pragma unroll
+      for (int j = 0; j < NBL; ++j) {
+#pragma unroll
+        for (int m = 0; m < RL; ++m) {
+          const int n = t * NBL + j + m * SL;
+          if (!G::kept(n)) continue;
+          const int ix = G::compact_of(n);
+          const size_t pt = (size_t(ix) * G::K + iy) * cpz + iz;
+          const int sl = j * RL + m;
+          const R kx = a.ks * R(G::kint(ix));
+          const cpx<R> v[6] = {o[0][sl], o[1][sl], o[2][sl], o[3][sl], o[4][sl], o[5][sl]};
+          nusum += adj_point(a, pt, cs, kx, ky, kz, iz, v);
+        }
+      }
+    }
+    if (a.nupart) {
+      __syncthreads();  // every thread is done reading buf
+      double* red = reinterpret_cast<double*>(buf);
+      red[threadIdx.x] = nusum;
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int i = 0; i < int(blockDim.x); ++i) s += red[i];
+        a.nupart[size_t(blockIdx.y) * gridDim.x + blockIdx.x] += s;
       }
     }
   }
 };
 
+// Register budget: three waves per SIMD for the FP64 hydro kernel (its few
+// spills cost less than the lost occupancy, measured); the others keep the
+// compiler's allocation (capping them spills more than it gains).
+template <typename R, int KIND>
+constexpr int xr_waves_per_eu() {
+  return (sizeof(R) == 8 && KIND == XR_HYDRO) ? 3 : 1;
+}
+
 template <typename R, int N, bool MHD>
-__global__ __launch_bounds__((CfgXR<R, N>::NT)) void k_fwd_xr(const cpx<R>* __restrict__ in, StageArgs<R> a,
+__global__ __launch_bounds__((CfgXR<R, N>::NT))
+__attribute__((amdgpu_waves_per_eu(xr_waves_per_eu<R, MHD ? XR_MHD : XR_HYDRO>()))) void k_fwd_xr(const cpx<R>* __restrict__ in, StageArgs<R> a,
                                                              const cpx<R>* __restrict__ tw) {
-  using B = XRBody<R, N, MHD>;
+  using B = XRBody<R, N, MHD ? XR_MHD : XR_HYDRO>;
+  __shared__ cpx<R> sm[B::SMEM];
+  B::run(sm, blockIdx.x, blockIdx.y, in, a, tw);
+}
+
+// Forward x of one stage VJP with its adjoint epilogue on register-held
+// lines (grids of N >= 128; one 128-byte line of kz columns per block).
+template <typename R, int N>
+__global__ __launch_bounds__((CfgXR<R, N>::NT)) __attribute__((amdgpu_waves_per_eu(xr_waves_per_eu<R, XR_ADJ>())))
+void k_fwd_xr_adj(const cpx<R>* __restrict__ in, StageArgs<R> a,
+                                                                 const cpx<R>* __restrict__ tw) {
+  using B = XRBody<R, N, XR_ADJ>;
   __shared__ cpx<R> sm[B::SMEM];
   B::run(sm, blockIdx.x, blockIdx.y, in, a, tw);
 }
@@ -1418,15 +1539,21 @@ __global__ __launch_bounds__((CfgXR<R, N>::NT)) void k_fwd_xr(const cpx<R>* __re
 // ----------------------------------------------------------------- adjoint helpers
 // Cotangent of the final state for J = E (objective 0) or Z (1), as the
 // unnormalised spectrum of the physical gradient: dE/du = u / N^3,
-// dZ/du = curl curl u / N^3 = (k^2 u - k (k.u)) / N^3.
+// dZ/du = curl curl u / N^3 = (k^2 u - k (k.u)) / N^3.  x and lam have row
+// pitch cpz >= KZ (the padding columns of lam are zeroed).
 template <typename R, int N>
 __global__ __launch_bounds__(256) void k_obj_grad(const cpx<R>* __restrict__ x, cpx<R>* __restrict__ lam,
-                                                  int objective, R ks) {
+                                                  int objective, R ks, int cpz) {
   using G = Geo<N>;
-  const size_t cs = size_t(G::K) * G::K * G::KZ;
+  const size_t cs = size_t(G::K) * G::K * cpz;
   const size_t pt = size_t(blockIdx.x) * 256 + threadIdx.x;
   if (pt >= cs) return;
-  const int iz = int(pt % G::KZ), iy = int((pt / G::KZ) % G::K), ix = int(pt / (size_t(G::KZ) * G::K));
+  const int iz = int(pt % cpz), iy = int((pt / cpz) % G::K), ix = int(pt / (size_t(cpz) * G::K));
+  if (iz >= G::KZ) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) lam[c * cs + pt] = {R(0), R(0)};
+    return;
+  }
   const R sc = R(1) / (R(N) * R(N) * R(N));
   cpx<R> v[3] = {x[pt], x[cs + pt], x[2 * cs + pt]};
   if (objective == 1) {
@@ -1441,17 +1568,21 @@ __global__ __launch_bounds__(256) void k_obj_grad(const cpx<R>* __restrict__ x, 
   for (int c = 0; c < 3; ++c) lam[c * cs + pt] = cscale(v[c], sc);
 }
 
-// Physical-cotangent spectrum -> gradient w.r.t. the stored compact entries
-// (d/dRe + i d/dIm): weight (1 at kz = 0, 2 otherwise) / N^3.
+// Physical-cotangent spectrum (row pitch cpz) -> gradient w.r.t. the stored
+// compact entries (dense pitch KZ; d/dRe + i d/dIm): weight (1 at kz = 0,
+// 2 otherwise) / N^3.
 template <typename R, int N>
-__global__ __launch_bounds__(256) void k_adj_final(const cpx<R>* __restrict__ lam, cpx<R>* __restrict__ g) {
+__global__ __launch_bounds__(256) void k_adj_final(const cpx<R>* __restrict__ lam, cpx<R>* __restrict__ g, int cpz) {
   using G = Geo<N>;
-  const size_t cs = size_t(G::K) * G::K * G::KZ;
+  const size_t cs = size_t(G::K) * G::K * G::KZ, csl = size_t(G::K) * G::K * cpz;
   const size_t pt = size_t(blockIdx.x) * 256 + threadIdx.x;
   if (pt >= cs) return;
-  const R w = (pt % G::KZ == 0 ? R(1) : R(2)) / (R(N) * R(N) * R(N));
+  const size_t row = pt / G::KZ;
+  const int iz = int(pt % G::KZ);
+  const size_t src = row * cpz + iz;
+  const R w = (iz == 0 ? R(1) : R(2)) / (R(N) * R(N) * R(N));
 #pragma unroll
-  for (int c = 0; c < 3; ++c) g[c * cs + pt] = cscale(lam[c * cs + pt], w);
+  for (int c = 0; c < 3; ++c) g[c * cs + pt] = cscale(lam[c * csl + src], w);
 }
 
 // ----------------------------------------------------------------- diagnostics

@@ -246,9 +246,16 @@ struct Launch {
     }
   }
   static int nmax() { return (N / CfgZM<R, N, ZM_MAXABS, 0>::LZ) * N; }
-  // RK4 stage variant 1 of N >= 128 (hydro and MHD) is the register-epilogue kernel.
+  // RK4 stage variant 1 of N >= 128 (hydro and MHD) is the register-epilogue
+  // kernel; so is every adjoint launch on those grids.
   static int fwd_x(int mode, int var, const cpx<R>* in, const StageArgs<R>& a, const cpx<R>* tw, hipStream_t s) {
     if constexpr (N >= 128) {
+      if (mode == FM_ADJ) {
+        using C = CfgXR<R, N>;
+        hipLaunchKernelGGL((k_fwd_xr_adj<R, N>), dim3(cdiv(G::KZ, C::L), G::K), dim3(C::NT), 0, s, in, a, tw);
+        TG_LAUNCHED();
+        return TG_OK;
+      }
       if ((mode == FM_HYDRO_STAGE || mode == FM_MHD_STAGE) && var == 1) {
         using C = CfgXR<R, N>;
         const dim3 grid(cdiv(G::KZ, C::L), G::K), blk(C::NT);
@@ -276,12 +283,10 @@ struct Launch {
       return TG_OK;
     });
   }
-  // blocks of the forward-x pass (slots of the adjoint nu-gradient partials)
-  static int fwd_x_blocks(int var, bool six) {
-    return with_variant<N>(var, [&](auto vc) {
-      using C = CfgXF<R, N, decltype(vc)::value>;
-      return cdiv(G::KZ, C::LINES / (six ? 6 : 3)) * G::K;
-    });
+  // blocks of the adjoint forward-x pass (slots of the nu-gradient partials)
+  static int adj_x_blocks() {
+    if constexpr (N >= 128) return cdiv(G::KZ, CfgXR<R, N>::L) * G::K;
+    else return cdiv(G::KZ, CfgXF<R, N, 0>::LINES / 6) * G::K;
   }
   static int diag(const cpx<R>* su, const cpx<R>* sb, double* part, int nblk, double ks, hipStream_t s) {
     hipLaunchKernelGGL((k_diag<R, N>), dim3(nblk), dim3(DIAG_NT), 0, s, su, sb, part, ks);
@@ -395,13 +400,25 @@ struct Ops {
     const size_t vec = rows * KZP;  // padded vector (3 components)
     cpx<R>* u0 = w.U0;
     cpx<R>* b0 = w.U0 + vec;
+    const size_t e = sizeof(cpx<R>);
+    TG_HIP(hipMemcpy2DAsync(u0, KZP * e, u, KZ * e, KZ * e, rows, hipMemcpyDeviceToDevice, s));
+    if (p->mhd) TG_HIP(hipMemcpy2DAsync(b0, KZP * e, b, KZ * e, KZ * e, rows, hipMemcpyDeviceToDevice, s));
+    TG_RET(rk4_padded(p, u0, b0, nsteps, dt, nu, eta, lor, ws, s));
+    TG_HIP(hipMemcpy2DAsync(u, KZ * e, u0, KZP * e, KZ * e, rows, hipMemcpyDeviceToDevice, s));
+    if (p->mhd) TG_HIP(hipMemcpy2DAsync(b, KZ * e, b0, KZP * e, KZ * e, rows, hipMemcpyDeviceToDevice, s));
+    return TG_OK;
+  }
+  // nsteps RK4 steps on line-padded state vectors u0 (b0) in place; the
+  // stage input and accumulator live in the workspace.
+  static int rk4_padded(const tg_plan* p, cpx<R>* u0, cpx<R>* b0, int nsteps, double dt, double nu, double eta,
+                        double lor, void* ws, hipStream_t s) {
+    Ptrs<R> w = carve<R>(p, ws);
+    constexpr int KZP = kzp<R, N>();
+    const size_t vec = size_t(3) * p->K * p->K * KZP;
     cpx<R>* u1 = w.U1;
     cpx<R>* b1 = w.U1 + vec;
     cpx<R>* ua = w.Acc;
     cpx<R>* ba = w.Acc + vec;
-    const size_t e = sizeof(cpx<R>);
-    TG_HIP(hipMemcpy2DAsync(u0, KZP * e, u, KZ * e, KZ * e, rows, hipMemcpyDeviceToDevice, s));
-    if (p->mhd) TG_HIP(hipMemcpy2DAsync(b0, KZP * e, b, KZ * e, KZ * e, rows, hipMemcpyDeviceToDevice, s));
     const int fm = p->mhd ? FM_MHD_STAGE : FM_HYDRO_STAGE;
     for (int step = 0; step < nsteps; ++step) {
       for (int st = 0; st < 4; ++st) {
@@ -418,8 +435,6 @@ struct Ops {
         TG_RET(rhs_passes<R, N>(p, w, a.us, a.bs, fm, a, R(lor), s));
       }
     }
-    TG_HIP(hipMemcpy2DAsync(u, KZ * e, u0, KZP * e, KZ * e, rows, hipMemcpyDeviceToDevice, s));
-    if (p->mhd) TG_HIP(hipMemcpy2DAsync(b, KZ * e, b0, KZP * e, KZ * e, rows, hipMemcpyDeviceToDevice, s));
     return TG_OK;
   }
   // One vector-Jacobian product of the hydro RHS at x for the cotangent a
@@ -430,8 +445,10 @@ struct Ops {
                         hipStream_t s, const cpx<R>* uw = nullptr) {
     using Lc = Launch<R, N>;
     const cpx<R>* tw = static_cast<const cpx<R>*>(p->tw);
+    const int cpz = a.cpz ? a.cpz : Geo<N>::KZ;  // row pitch of x, a and the epilogue vectors
     if (uw) {
-      TG_TIMED(TG_PASS_INV_X, Lc::inv_x(XM_ADJQ, p->var[TG_PASS_INV_X], a.a, nullptr, w.A, tw, R(p->ks), s, a.alpha));
+      TG_TIMED(TG_PASS_INV_X, Lc::inv_x(XM_ADJQ, p->var[TG_PASS_INV_X], a.a, nullptr, w.A, tw, R(p->ks), s, a.alpha,
+                                        cpz));
       TG_TIMED(TG_PASS_INV_Y, Lc::y(+1, p->var[TG_PASS_INV_Y], 3, w.A, w.B, tw, s));
       TG_TIMED(TG_PASS_Z, Lc::z_adj(uw, w.B, w.A, tw, s));
       TG_TIMED(TG_PASS_FWD_Y, Lc::y(-1, p->var[TG_PASS_FWD_Y], 6, w.A, w.B, tw, s));
@@ -440,7 +457,8 @@ struct Ops {
     }
     // the 8-points-per-thread tile (variant 1, where compiled) keeps the six
     // state and cotangent components in registers
-    TG_TIMED(TG_PASS_INV_X, Lc::inv_x(XM_ADJ, num_variants<N>() > 1 ? 1 : 0, x, a.a, w.A, tw, R(p->ks), s, a.alpha));
+    TG_TIMED(TG_PASS_INV_X, Lc::inv_x(XM_ADJ, num_variants<N>() > 1 ? 1 : 0, x, a.a, w.A, tw, R(p->ks), s, a.alpha,
+                                      cpz));
     TG_TIMED(TG_PASS_INV_Y, Lc::y(+1, p->var[TG_PASS_INV_Y], 9, w.A, w.B, tw, s));
     TG_TIMED(TG_PASS_Z, Lc::z(ZM_ADJ, 0, w.B, w.A, nullptr, nullptr, tw, R(0), s));
     TG_TIMED(TG_PASS_FWD_Y, Lc::y(-1, p->var[TG_PASS_FWD_Y], 6, w.A, w.B, tw, s));
@@ -456,7 +474,8 @@ struct Ops {
     return vjp_passes(p, w, x, a, s);
   }
   // Adjoint workspace: cur | lamp | lacc | abuf | nck checkpoints | 4 seg
-  // stage states | nu-gradient partials, and with more than one segment a
+  // stage states | nu-gradient partials, all vectors with line-padded rows
+  // (pitch kzp, as in the RK4 loop), and with more than one segment a
   // second set for the recompute that overlaps the reverse sweep: cur2 |
   // 4 seg stage states | RHS workspace.
   // On grids with the register adjoint z pass, each stage buffer has a
@@ -467,11 +486,13 @@ struct Ops {
     cpx<R>* slots; cpx<R>* slots2;
     size_t vstride, sstride; int nck, nblk;
   };
+  // one padded 3-component vector
+  static size_t vec_bytes(const tg_plan* p) { return align256(size_t(3) * p->K * p->K * kzp<R, N>() * sizeof(cpx<R>)); }
   static size_t slot_bytes(const tg_plan* p) { return align256(size_t(6) * N * N * kzp<R, N>() * sizeof(cpx<R>)); }
   static size_t nupart_bytes(const tg_plan* p) { return align256(size_t(p->K) * p->KZ * 8 * sizeof(double)); }
   static AdjLayout adj_layout(const tg_plan* p, int nsteps, int seg, void* aws) {
     AdjLayout L;
-    const size_t vb = align256(3 * p->compact * sizeof(cpx<R>));
+    const size_t vb = vec_bytes(p);
     L.nck = (nsteps + seg - 1) / seg;
     char* base = static_cast<char*>(aws);
     L.cur = reinterpret_cast<cpx<R>*>(base);
@@ -491,7 +512,7 @@ struct Ops {
     L.slots2 = (adj_reg_z<N>() && L.nck > 1) ? reinterpret_cast<cpx<R>*>(third + 4 * size_t(seg) * sb) : nullptr;
     L.sstride = sb / sizeof(cpx<R>);
     L.vstride = vb / sizeof(cpx<R>);
-    L.nblk = Launch<R, N>::fwd_x_blocks(0, true);
+    L.nblk = Launch<R, N>::adj_x_blocks();
     return L;
   }
   // `len` RK4 steps from cur keeping the four stage states of step j in
@@ -500,7 +521,7 @@ struct Ops {
                                   double nu, void* ws, hipStream_t s, cpx<R>* slots = nullptr,
                                   size_t sstride = 0) {
     Ptrs<R> w = carve<R>(p, ws);
-    const size_t bytes = 3 * p->compact * sizeof(cpx<R>);
+    const size_t bytes = size_t(3) * p->K * p->K * kzp<R, N>() * sizeof(cpx<R>);
     for (int j = 0; j < len; ++j) {
       cpx<R>* S = st + size_t(4 * j) * vstride;
       TG_HIP(hipMemcpyAsync(S, cur, bytes, hipMemcpyDeviceToDevice, s));
@@ -511,6 +532,7 @@ struct Ops {
         a.unext = stg == 3 ? cur : S + size_t(stg + 1) * vstride;
         a.uacc = w.Acc;
         a.dt = R(dt); a.nu = R(nu); a.ks = R(p->ks); a.stage = stg;
+        a.cpz = kzp<R, N>();
         cpx<R>* slot = slots ? slots + size_t(4 * j + stg) * sstride : nullptr;
         TG_RET(rhs_passes<R, N>(p, w, a.us, nullptr, FM_HYDRO_STAGE, a, R(0), s, slot));
       }
@@ -522,20 +544,24 @@ struct Ops {
   static int adj_forward(const tg_plan* p, const cpx<R>* x0, int nsteps, double dt, double nu, int objective,
                          int seg, double* J, void* ws, void* aws, hipStream_t s) {
     AdjLayout L = adj_layout(p, nsteps, seg, aws);
-    const size_t bytes = 3 * p->compact * sizeof(cpx<R>);
-    TG_HIP(hipMemcpyAsync(L.cur, x0, bytes, hipMemcpyDeviceToDevice, s));
+    constexpr int KZP = kzp<R, N>(), KZ = Geo<N>::KZ;
+    const size_t rows = size_t(3) * p->K * p->K, e = sizeof(cpx<R>);
+    const size_t bytes = rows * KZP * e;
+    TG_HIP(hipMemcpy2DAsync(L.cur, KZP * e, x0, KZ * e, KZ * e, rows, hipMemcpyDeviceToDevice, s));
     for (int c = 0; c < L.nck; ++c) {
       TG_HIP(hipMemcpyAsync(L.ck + c * L.vstride, L.cur, bytes, hipMemcpyDeviceToDevice, s));
       const int len = std::min(seg, nsteps - c * seg);
-      if (c < L.nck - 1) TG_RET(rk4(p, L.cur, nullptr, len, dt, nu, 0.0, -1.0, ws, s));
+      if (c < L.nck - 1) TG_RET(rk4_padded(p, L.cur, nullptr, len, dt, nu, 0.0, -1.0, ws, s));
       else TG_RET(steps_keeping_stages(p, L.cur, L.st, L.vstride, len, dt, nu, ws, s, L.slots,
                                        L.sstride));  // the reverse sweep starts here
     }
+    // diagnostics read the dense layout: a copy of the final state in abuf
+    TG_HIP(hipMemcpy2DAsync(L.abuf, KZ * e, L.cur, KZP * e, KZ * e, rows, hipMemcpyDeviceToDevice, s));
     double d[NDIAG + 2];
-    TG_RET(diagnostics(p, L.cur, nullptr, 0, d, ws, s));
+    TG_RET(diagnostics(p, L.abuf, nullptr, 0, d, ws, s));
     *J = objective == 0 ? d[0] : d[1];
-    hipLaunchKernelGGL((k_obj_grad<R, N>), dim3(cdiv(int(p->compact), 256)), dim3(256), 0, s, L.cur, L.lamp,
-                       objective, R(p->ks));
+    hipLaunchKernelGGL((k_obj_grad<R, N>), dim3(cdiv(int(rows / 3 * KZP), 256)), dim3(256), 0, s, L.cur, L.lamp,
+                       objective, R(p->ks), KZP);
     TG_LAUNCHED();
     return TG_OK;
   }
@@ -547,7 +573,8 @@ struct Ops {
                           double* dJdnu, void* ws, void* aws, hipStream_t s) {
     Ptrs<R> w = carve<R>(p, ws);
     AdjLayout L = adj_layout(p, nsteps, seg, aws);
-    const size_t bytes = 3 * p->compact * sizeof(cpx<R>);
+    constexpr int KZP = kzp<R, N>();
+    const size_t bytes = size_t(3) * p->K * p->K * KZP * sizeof(cpx<R>);
     TG_HIP(hipMemsetAsync(L.nupart, 0, L.nblk * sizeof(double), s));
     cpx<R>* lamp = L.lamp;
     cpx<R>* lacc = L.lacc;
@@ -590,7 +617,7 @@ struct Ops {
         cpx<R>* S = stc + size_t(4 * j) * L.vstride;
         for (int sidx = 3; sidx >= 0; --sidx) {
           StageArgs<R> a{};
-          a.nu = R(nu); a.ks = R(p->ks);
+          a.nu = R(nu); a.ks = R(p->ks); a.cpz = KZP;
           a.lamp = lamp; a.lacc = lacc; a.xs = S + size_t(sidx) * L.vstride;
           a.nupart = L.nupart;
           if (sidx == 3) { a.a = lamp; a.alpha = cl[3]; a.first = 1; }
@@ -608,7 +635,7 @@ struct Ops {
         TG_HIP(hipStreamWaitEvent(s2, ev_rev, 0));
       }
     }
-    hipLaunchKernelGGL((k_adj_final<R, N>), dim3(cdiv(int(p->compact), 256)), dim3(256), 0, s, lamp, grad);
+    hipLaunchKernelGGL((k_adj_final<R, N>), dim3(cdiv(int(p->compact), 256)), dim3(256), 0, s, lamp, grad, KZP);
     TG_LAUNCHED();
     std::vector<double> part(L.nblk);
     TG_HIP(hipMemcpyAsync(part.data(), L.nupart, L.nblk * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -908,12 +935,12 @@ int tg_vjp(tg_plan* p, const void* x_hat, const void* a_hat, double nu, void* ou
 int tg_adjoint_workspace(tg_plan* p, int nsteps, int seg, long long* bytes) {
   TG_RET(check_plan(p));
   if (nsteps < 1 || seg < 1 || !bytes) return fail(TG_ERR_ARG, "tg_adjoint_workspace: need nsteps, seg >= 1");
-  const size_t vb = align256(3 * p->compact * p->cb);
+  const size_t kzpad = p->prec == 64 ? size_t(kzp_rt<double>(p->n)) : size_t(kzp_rt<float>(p->n));
+  const size_t vb = align256(3 * size_t(p->K) * p->K * kzpad * p->cb);  // line-padded vector
   const size_t nck = (nsteps + seg - 1) / seg;
   size_t b = (4 + nck + 4 * size_t(seg)) * vb + align256(size_t(p->K) * p->KZ * 8 * sizeof(double));
   if (nck > 1) b += (1 + 4 * size_t(seg)) * vb + align256(p->wsBytes);  // overlapped recompute
   if (adj_reg_z_rt(p->n)) {  // kept inverse-y output (u, w) per stage, per stage buffer
-    const size_t kzpad = p->prec == 64 ? size_t(kzp_rt<double>(p->n)) : size_t(kzp_rt<float>(p->n));
     const size_t sb = align256(6 * size_t(p->n) * p->n * kzpad * p->cb);
     b += (nck > 1 ? 2 : 1) * 4 * size_t(seg) * sb;
   }
