@@ -87,6 +87,27 @@ def test_gradient_matches_torch_autograd(kind):
     assert abs(nug.grad.item() - gnu.item()) < 1e-10 * abs(gnu.item()), (nug.grad.item(), gnu.item())
 
 
+def test_gradient_radix12_grid_matches_torch_autograd():
+    """N = 48 (radix-12 plans, not a power of two): the VJPs transform the
+    state and the cotangent (no kept stage fields), the workspace vectors
+    use the line-padded pitch (KZ = 17 -> 24), and seg = 2 over 6 steps
+    recomputes two segments on the overlapping stream."""
+    N, nu, dt, n, seg = 48, 0.02, 0.01, 6, 2
+    x = _state(N, 11)
+    c = torch.tensor(x, requires_grad=True)
+    nut = torch.tensor(nu, dtype=torch.float64, requires_grad=True)
+    J = ad.final_objective(c, nut, n, dt, N, "energy")
+    gc, gnu = torch.autograd.grad(J, (c, nut))
+    p = _plan(N)
+    aws = p.adjoint_workspace(n, seg)
+    Jg = p.adjoint_forward(torch.tensor(x, device="cuda"), n, dt, nu, 0, seg, aws)
+    g, dnu = p.adjoint_backward(n, dt, nu, seg, aws)
+    assert abs(Jg - J.item()) < 1e-13 * abs(J.item())
+    err = np.linalg.norm(g.cpu().numpy() - gc.numpy()) / np.linalg.norm(gc.numpy())
+    assert err < 1e-11, err
+    assert abs(dnu - gnu.item()) < 1e-10 * abs(gnu.item()), (dnu, gnu.item())
+
+
 @pytest.mark.parametrize("seg", [3, 10])
 def test_gradient_with_kept_stage_fields_matches_torch_autograd(seg):
     """N = 32 runs the register adjoint z pass with each stage's kept
