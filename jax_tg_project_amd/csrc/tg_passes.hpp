@@ -843,6 +843,65 @@ __device__ __forceinline__ void zw_load(cpx<R>* v, const cpx<R>* __restrict__ sr
   }
 }
 
+// zw_load in two phases for lines whose first stage holds one point per
+// radix slot (NB1 == 1, TPL a power of two): zw_fetch issues the loads of
+// the slots m < R1/2 into ra/rb, zw_combine forms the packed sequence and
+// its mirror half exactly as zw_load does.  A caller fetches several
+// sequences before combining any, so their loads are in flight together
+// instead of one sequence per memory round trip.
+template <typename R, int N, class C, int FA, int FB>
+__device__ __forceinline__ void zw_fetch(cpx<R>* ra, cpx<R>* rb, const cpx<R>* __restrict__ src, size_t NNKZ,
+                                         int t) {
+  using IP = typename C::IP;
+  constexpr int E = C::E, TPL = C::TPL, KZ = Geo<N>::KZ;
+  constexpr int R1 = FirstRadix<IP>::value, NB1 = E / R1, S1 = N / R1;
+  static_assert(NB1 == 1 && (TPL & (TPL - 1)) == 0, "two-phase load needs one point per radix slot");
+#pragma unroll
+  for (int m = 0; m < R1 / 2; ++m) {
+    ra[m] = rb[m] = {R(0), R(0)};
+    if (!band_zero<N, KZ>(m * S1, m * S1 + S1 - 1)) {
+      const int k = t + m * S1;
+      const int kk = k < KZ ? k : KZ - 1;
+      ra[m] = gld<nt_mid<N>()>(&src[FA * NNKZ + kk]);
+      if constexpr (FB >= 0) rb[m] = gld<nt_mid<N>()>(&src[FB * NNKZ + kk]);
+    }
+  }
+}
+
+template <typename R, int N, class C>
+__device__ __forceinline__ void zw_combine(cpx<R>* v, const cpx<R>* ra, const cpx<R>* rb, int t, int lane) {
+  using IP = typename C::IP;
+  constexpr int TPL = C::TPL, KZ = Geo<N>::KZ;
+  constexpr int R1 = FirstRadix<IP>::value, S1 = N / R1;
+  const int partner = (lane & ~(TPL - 1)) | ((TPL - t) & (TPL - 1));
+  cpx<R> w[R1 / 2];  // conj(a) + i conj(b): the value at N - k
+#pragma unroll
+  for (int m = 0; m < R1 / 2; ++m) {
+    cpx<R> z = {R(0), R(0)}, zm = {R(0), R(0)};
+    if (!band_zero<N, KZ>(m * S1, m * S1 + S1 - 1)) {
+      const int k = t + m * S1;
+      cpx<R> a = ra[m], b = rb[m];
+      if (k == 0) { a.y = R(0); b.y = R(0); }
+      if (k < KZ) {
+        z = {a.x - b.y, a.y + b.x};
+        zm = {a.x + b.y, b.x - a.y};
+      }
+    }
+    v[m] = z;
+    w[m] = zm;
+  }
+#pragma unroll
+  for (int m = R1 / 2; m < R1; ++m) {
+    cpx<R> z = {R(0), R(0)};
+    if (!band_zero<N, KZ>(m * S1, m * S1 + S1 - 1)) {
+      const cpx<R> f = lane_get(w[R1 - 1 - m], partner);
+      const cpx<R> own = (m == R1 / 2) ? cpx<R>{R(0), R(0)} : w[(R1 - m) % R1];
+      z = t == 0 ? own : f;
+    }
+    v[m] = z;
+  }
+}
+
 // Store the kept kz of the two real fields packed in the forward transform
 // x (output order of plan FP) into fields FA (real part) and FB: they are
 // split by Hermitian symmetry from the pair (k, N - k), through the line
@@ -965,11 +1024,17 @@ struct ZWAdjBody {
     cpx<R>* dst = out + line * KZP;
 
     cpx<R> v[5][E];
-    zw_load<R, N, C, 0, 1>(v[0], src, NNKZ, t, lane);
-    zw_load<R, N, C, 2, 3>(v[1], src, NNKZ, t, lane);
-    zw_load<R, N, C, 4, 5>(v[2], src, NNKZ, t, lane);
-    zw_load<R, N, C, 0, 1>(v[3], qsrc, NNKZ, t, lane);
-    zw_load<R, N, C, 2, -1>(v[4], qsrc, NNKZ, t, lane);
+    {
+      constexpr int H = FirstRadix<IP>::value / 2;
+      cpx<R> ra[5][H], rb[5][H];  // every load of the five sequences first
+      zw_fetch<R, N, C, 0, 1>(ra[0], rb[0], src, NNKZ, t);
+      zw_fetch<R, N, C, 2, 3>(ra[1], rb[1], src, NNKZ, t);
+      zw_fetch<R, N, C, 4, 5>(ra[2], rb[2], src, NNKZ, t);
+      zw_fetch<R, N, C, 0, 1>(ra[3], rb[3], qsrc, NNKZ, t);
+      zw_fetch<R, N, C, 2, -1>(ra[4], rb[4], qsrc, NNKZ, t);
+#pragma unroll
+      for (int p = 0; p < 5; ++p) zw_combine<R, N, C>(v[p], ra[p], rb[p], t, lane);
+    }
     __syncthreads();  // twiddle rows staged
 #pragma unroll
     for (int p = 0; p < 5; ++p) reg_fft<R, N, E, +1, PS>(v[p], t, lb, tsi, IP{});
