@@ -1088,12 +1088,18 @@ struct ZWMhdBody {
     cpx<R>* dst = out + line * KZP;
 
     cpx<R> v[6][E];
-    zw_load<R, N, C, 0, 3>(v[0], src, NNKZ, t, lane);
-    zw_load<R, N, C, 1, 4>(v[1], src, NNKZ, t, lane);
-    zw_load<R, N, C, 2, 5>(v[2], src, NNKZ, t, lane);
-    zw_load<R, N, C, 6, 9>(v[3], src, NNKZ, t, lane);
-    zw_load<R, N, C, 7, 10>(v[4], src, NNKZ, t, lane);
-    zw_load<R, N, C, 8, 11>(v[5], src, NNKZ, t, lane);
+    {
+      constexpr int H = FirstRadix<IP>::value / 2;
+      cpx<R> ra[6][H], rb[6][H];  // every load of the six sequences first
+      zw_fetch<R, N, C, 0, 3>(ra[0], rb[0], src, NNKZ, t);
+      zw_fetch<R, N, C, 1, 4>(ra[1], rb[1], src, NNKZ, t);
+      zw_fetch<R, N, C, 2, 5>(ra[2], rb[2], src, NNKZ, t);
+      zw_fetch<R, N, C, 6, 9>(ra[3], rb[3], src, NNKZ, t);
+      zw_fetch<R, N, C, 7, 10>(ra[4], rb[4], src, NNKZ, t);
+      zw_fetch<R, N, C, 8, 11>(ra[5], rb[5], src, NNKZ, t);
+#pragma unroll
+      for (int p = 0; p < 6; ++p) zw_combine<R, N, C>(v[p], ra[p], rb[p], t, lane);
+    }
     __syncthreads();  // twiddle rows staged
 #pragma unroll
     for (int p = 0; p < 6; ++p) reg_fft<R, N, E, +1, PS>(v[p], t, lb, tsi, IP{});
