@@ -366,18 +366,15 @@ struct YCurlBody {
   static constexpr int SMEM = N * C::L + N;
   static constexpr size_t AF = size_t(N) * G::K * kzp<R, N>();  // field stride of A
   static constexpr size_t BF = size_t(N) * N * kzp<R, N>();     // field stride of B
-  // (bx, x, f, vec): kz tile, x plane, output field 0-5, vector.  A[5 vec ..]
-  // -> B[6 vec + f].
-  __device__ __forceinline__ static void run(cpx<R>* sm, int bx, int x, int f, int vec,
-                                             const cpx<R>* __restrict__ in, cpx<R>* __restrict__ out,
-                                             const cpx<R>* __restrict__ tw, R ks) {
+  // Output field F of one (x, kz tile).  F is a template argument so the
+  // first stage's loads carry no per-point branch: all of them issue before
+  // the curl arithmetic waits on any.
+  template <int F>
+  __device__ __forceinline__ static void field(cpx<R>* sm, int bx, const cpx<R>* __restrict__ a,
+                                               cpx<R>* __restrict__ b, const cpx<R>* __restrict__ tws, R ks) {
     constexpr int L = C::L, E = C::E, KZP = kzp<R, N>();
-    cpx<R>* buf = sm;
-    cpx<R>* tws = sm + N * L;
+    constexpr bool NT = nt_mid<N>();
     const int iz0 = bx * L;
-    const cpx<R>* a = in + size_t(5 * vec) * AF + size_t(x) * G::K * KZP;
-    cpx<R>* b = out + (size_t(6 * vec) + f) * BF + size_t(x) * N * KZP;
-    stage_twiddles<R, N>(tws, tw);
     auto ld = [&](int n, int ll, int) -> cpx<R> {
       const int izr = iz0 + ll;
       const bool colok = izr < G::KZ;
@@ -386,25 +383,45 @@ struct YCurlBody {
       const size_t pt = size_t(iy) * KZP + (colok ? izr : G::KZ - 1);
       const R ky = ks * R(G::kint(iy)), kz = ks * R(izr);
       cpx<R> v;
-      if (f < 3) {
-        v = gld<false>(&a[f * AF + pt]);
-      } else if (f == 3) {
+      if constexpr (F < 3) {
+        v = gld<false>(&a[F * AF + pt]);
+      } else if constexpr (F == 3) {  // w_x = i (ky v_z - kz v_y)
         const cpx<R> y = gld<false>(&a[AF + pt]), z = gld<false>(&a[2 * AF + pt]);
         v = cmuli(cpx<R>{ky * z.x - kz * y.x, ky * z.y - kz * y.y});
-      } else {
+      } else {  // w_y = i kz v_x - D_z,  w_z = D_y - i ky v_x
         const cpx<R> vx = gld<false>(&a[pt]);
-        const cpx<R> d = gld<nt_mid<N>()>(&a[(f == 4 ? 4 : 3) * AF + pt]);
-        const R k = f == 4 ? kz : ky;
+        const cpx<R> d = gld<NT>(&a[(F == 4 ? 4 : 3) * AF + pt]);
+        const R k = F == 4 ? kz : ky;
         const cpx<R> t = cmuli(cpx<R>{k * vx.x, k * vx.y});
-        v = f == 4 ? cpx<R>{t.x - d.x, t.y - d.y} : cpx<R>{d.x - t.x, d.y - t.y};
+        v = F == 4 ? cpx<R>{t.x - d.x, t.y - d.y} : cpx<R>{d.x - t.x, d.y - t.y};
       }
       return ok ? v : cpx<R>{R(0), R(0)};
     };
     auto st = [&](int n, int ll, cpx<R> v, int) {
       const int iz = iz0 + ll;
-      if (iz < G::KZ) gst<nt_mid<N>()>(&b[size_t(n) * KZP + iz], v);
+      if (iz < G::KZ) gst<NT>(&b[F * BF + size_t(n) * KZP + iz], v);
     };
-    fft_io<R, N, E, +1, 1, true>(buf, L, L, tws, ld, st);
+    fft_io<R, N, E, +1, 1, true>(sm, L, L, tws, ld, st);
+  }
+
+  // (bx, x, f, vec): kz tile, x plane, output field 0-5, vector.  A[5 vec ..]
+  // -> B[6 vec + f].
+  __device__ __forceinline__ static void run(cpx<R>* sm, int bx, int x, int f, int vec,
+                                             const cpx<R>* __restrict__ in, cpx<R>* __restrict__ out,
+                                             const cpx<R>* __restrict__ tw, R ks) {
+    constexpr int KZP = kzp<R, N>();
+    const cpx<R>* a = in + size_t(5 * vec) * AF + size_t(x) * G::K * KZP;
+    cpx<R>* b = out + size_t(6 * vec) * BF + size_t(x) * N * KZP;
+    cpx<R>* tws = sm + N * C::L;
+    stage_twiddles<R, N>(tws, tw);
+    switch (f) {
+      case 0: field<0>(sm, bx, a, b, tws, ks); break;
+      case 1: field<1>(sm, bx, a, b, tws, ks); break;
+      case 2: field<2>(sm, bx, a, b, tws, ks); break;
+      case 3: field<3>(sm, bx, a, b, tws, ks); break;
+      case 4: field<4>(sm, bx, a, b, tws, ks); break;
+      default: field<5>(sm, bx, a, b, tws, ks); break;
+    }
   }
 };
 
