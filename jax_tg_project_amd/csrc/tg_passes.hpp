@@ -973,9 +973,18 @@ struct ZWBody {
 
     // ---- the packed sequences u_p + i w_p
     cpx<R> v[3][E];
-    zw_load<R, N, C, 0, 3>(v[0], src, NNKZ, t, lane);
-    zw_load<R, N, C, 1, 4>(v[1], src, NNKZ, t, lane);
-    zw_load<R, N, C, 2, 5>(v[2], src, NNKZ, t, lane);
+    if constexpr (NB1 == 1 && (TPL & (TPL - 1)) == 0) {
+      cpx<R> ra[3][R1 / 2], rb[3][R1 / 2];  // every load of the three sequences first
+      zw_fetch<R, N, C, 0, 3>(ra[0], rb[0], src, NNKZ, t);
+      zw_fetch<R, N, C, 1, 4>(ra[1], rb[1], src, NNKZ, t);
+      zw_fetch<R, N, C, 2, 5>(ra[2], rb[2], src, NNKZ, t);
+#pragma unroll
+      for (int p = 0; p < 3; ++p) zw_combine<R, N, C>(v[p], ra[p], rb[p], t, lane);
+    } else {
+      zw_load<R, N, C, 0, 3>(v[0], src, NNKZ, t, lane);
+      zw_load<R, N, C, 1, 4>(v[1], src, NNKZ, t, lane);
+      zw_load<R, N, C, 2, 5>(v[2], src, NNKZ, t, lane);
+    }
     __syncthreads();  // twiddle rows staged
 #pragma unroll
     for (int p = 0; p < 3; ++p) reg_fft<R, N, E, +1, PS>(v[p], t, lb, tsi, IP{});
