@@ -232,6 +232,12 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-mhd", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--only", choices=("fp32", "ensemble", "mhd", "adjoint"),
+                    help="run one leg alone (for profiling) and print its JSON object")
+    ap.add_argument("--leg-steps", type=int, default=None,
+                    help="steps of the --only leg (default: the leg's workload)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="with --only: start with a 1 GiB device copy (counter calibration)")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="only run a 1 GiB device copy (counter calibration) and 3 tuned FP64 steps, "
                          "for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs")
@@ -261,6 +267,30 @@ def main():
                           "tile_variants": tuned}), flush=True)
         return
     K, W = args.steps, args.warmup
+    if args.only:
+        if args.calibrate_copy:
+            src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")  # 1 GiB
+            dst = torch.empty_like(src)
+            for _ in range(3):
+                dst.copy_(src)
+            torch.cuda.synchronize()
+            del src, dst
+        ls = args.leg_steps
+        if args.only == "fp32":
+            r = run_gpu(N, torch.float32, ls or K, W, dt, nu, tune=not args.no_tune)
+            out = {"fp32_steps_per_s": (ls or K) / r["wall"], "tile_variants_fp32": r["variants"],
+                   "passes_fp32": {k: {"ms_avg": v[0] / max(1, v[1]), "launches": v[1]}
+                                   for k, v in r["passes"].items()}}
+        elif args.only == "ensemble":
+            out = run_ensemble(64, ls or 200, 5e-3)
+        elif args.only == "mhd":
+            out = {"fp64": run_mhd(128, ls or 300, 2e-3, 1.0 / 200, 1.0 / 200, torch.float64),
+                   "fp32": run_mhd(128, ls or 300, 2e-3, 1.0 / 200, 1.0 / 200, torch.float32)}
+        else:
+            out = [run_adjoint(128, ls or 200, 5e-3, 1.0 / 400, d_, all_stages=False)
+                   for d_ in (torch.float64, torch.float32)]
+        print(json.dumps({"only": args.only, "result": out}), flush=True)
+        return
 
     r64 = run_gpu(N, torch.float64, K, W, dt, nu, tune=not args.no_tune)
     steps_s = K / r64["wall"]

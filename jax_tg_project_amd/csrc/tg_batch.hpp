@@ -9,6 +9,7 @@
 // body (the same code the single-instance kernels run) on its local block
 // index, with that body's exact block shape.
 #pragma once
+#include "tg_common.hpp"
 #include "tg_passes.hpp"
 
 namespace tg {
@@ -135,18 +136,51 @@ __attribute__((amdgpu_waves_per_eu(EnsBlock<R, N, PASS>::WAVES))) void kb_pass(c
   ens_run<R, N, PASS>(sm, b - prefix[lo], inst[lo], dt, stage);
 }
 
-// Host-side block counts (same formulas as EnsGrid).
+inline constexpr bool is_ens_size(int n) {
+  for (int s : kEnsSizes)
+    if (s == n) return true;
+  return false;
+}
+
+// Batched entry points of one (precision, N); n == 0 when N has no batched
+// kernels.  `stage` launches the five passes of RK4 stage `st` for the `cnt`
+// instances whose descriptors start at `inst`; dpre / hpre hold, on the
+// device and on the host, EP_COUNT prefix tables of cnt + 1 block offsets.
 template <typename R>
-inline int ens_blocks(int n, int pass) {
-  switch (n) {
-    case 16: return EnsGrid<R, 16>::blocks(pass);
-    case 32: return EnsGrid<R, 32>::blocks(pass);
-    case 48: return EnsGrid<R, 48>::blocks(pass);
-    case 64: return EnsGrid<R, 64>::blocks(pass);
-    case 96: return EnsGrid<R, 96>::blocks(pass);
-    case 128: return EnsGrid<R, 128>::blocks(pass);
+struct EnsTable {
+  int n;
+  int blocks[EP_COUNT];
+  int (*stage)(const EnsInst<R>* inst, const int* dpre, const int* hpre, int cnt, int st, double dt,
+               hipStream_t s);
+};
+
+template <typename R, int N, int PASS>
+int ens_launch(const EnsInst<R>* inst, const int* dpre, const int* hpre, int cnt, int st, double dt, hipStream_t s) {
+  const int nblk = hpre[PASS * (cnt + 1) + cnt];
+  hipLaunchKernelGGL((kb_pass<R, N, PASS>), dim3(nblk), dim3(EnsBlock<R, N, PASS>::NT), 0, s, inst,
+                     dpre + PASS * (cnt + 1), cnt, R(dt), st);
+  const hipError_t e = hipGetLastError();
+  return e == hipSuccess ? TG_OK : fail(TG_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+}
+
+template <typename R, int N>
+int ens_stage(const EnsInst<R>* inst, const int* dpre, const int* hpre, int cnt, int st, double dt, hipStream_t s) {
+  TG_RET(ens_launch<R, N, EP_INV_X>(inst, dpre, hpre, cnt, st, dt, s));
+  TG_RET(ens_launch<R, N, EP_INV_Y>(inst, dpre, hpre, cnt, st, dt, s));
+  TG_RET(ens_launch<R, N, EP_Z>(inst, dpre, hpre, cnt, st, dt, s));
+  TG_RET(ens_launch<R, N, EP_FWD_Y>(inst, dpre, hpre, cnt, st, dt, s));
+  return ens_launch<R, N, EP_FWD_X>(inst, dpre, hpre, cnt, st, dt, s);
+}
+
+template <typename R, int N>
+EnsTable<R> make_ens_table() {
+  EnsTable<R> t{};
+  if constexpr (is_ens_size(N)) {
+    t.n = N;
+    for (int p = 0; p < EP_COUNT; ++p) t.blocks[p] = EnsGrid<R, N>::blocks(p);
+    t.stage = &ens_stage<R, N>;
   }
-  return -1;
+  return t;
 }
 
 }  // namespace tg

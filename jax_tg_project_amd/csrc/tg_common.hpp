@@ -1,0 +1,68 @@
+// Host-side definitions shared by the translation units of libtgdns.so: the
+// plan object, error reporting and the launch-checking macros.  The kernels
+// of each (precision, grid size) are compiled in their own translation unit
+// (tg_inst.hip) and reached through a table of entry points (tg_ops.hpp).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/tgdns.h"
+
+// Grid sizes with compiled plans: X(n) for each.  The build compiles
+// tg_inst.hip once per (precision, size) of this list.
+#define TG_GRID_SIZES(X) X(4) X(8) X(16) X(32) X(48) X(64) X(96) X(128) X(256)
+
+namespace tg {
+
+// Record `msg` as the calling thread's last error and return `code`.
+int fail(int code, const std::string& msg);
+
+inline size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+inline int cdiv(int a, int b) { return (a + b - 1) / b; }
+
+}  // namespace tg
+
+#define TG_HIP(expr)                                                                      \
+  do {                                                                                    \
+    hipError_t e_ = (expr);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      return ::tg::fail(TG_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));  \
+  } while (0)
+
+#define TG_LAUNCHED()                                                                     \
+  do {                                                                                    \
+    hipError_t e_ = hipGetLastError();                                                    \
+    if (e_ != hipSuccess)                                                                 \
+      return ::tg::fail(TG_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e_)); \
+  } while (0)
+
+#define TG_RET(...)          \
+  do {                       \
+    int r_ = (__VA_ARGS__);  \
+    if (r_) return r_;       \
+  } while (0)
+
+struct tg_plan {
+  int n, prec, mhd;
+  int M, K, KZ;
+  double ks;          // 2 pi / L
+  void* tw;           // N complex twiddles exp(-2 pi i j / N) in plan precision
+  size_t cb;          // bytes per complex element
+  size_t compact;     // complex elements per compact component
+  // workspace layout (byte offsets)
+  size_t offA, offB, offU0, offU1, offAcc, offPart, offMax, offOut, wsBytes;
+  int nDiagBlk, nMax;
+  int var[TG_NPASS] = {0, 0, 0, 0, 0};  // tile variant per pass
+  // optional per-pass timing: pairs of events around each pass launch
+  bool timing = false;
+  std::vector<hipEvent_t> evpool;
+  std::vector<int> evpass;   // pass id of each recorded pair
+  size_t evused = 0;         // pairs recorded since the last read
+  // adjoint: private stream (default priority) on which the reverse sweep
+  // recomputes the next segment while the current one is reversed
+  hipStream_t aux = nullptr;
+  hipEvent_t adjev[3] = {nullptr, nullptr, nullptr};
+  const void* ops = nullptr;  // OpsTable<R> of (prec, n)
+};
