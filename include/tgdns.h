@@ -50,7 +50,8 @@ extern "C" {
 
 typedef struct tg_plan tg_plan;
 
-/* Library ABI version (monotone integer). */
+/* Library ABI version (monotone integer).  2: tg_diagnostics writes
+ * TG_DIAG_COUNT values, TG_OP_GRAD. */
 int tg_version(void);
 
 /* Grid sizes with compiled plans, written to out[0..cap); returns the count. */
@@ -77,14 +78,19 @@ int tg_forward(tg_plan* plan, const void* phys, void* spec, void* ws, void* stre
 #define TG_OP_FIELD 0 /* the field itself          -> phys [3][N][N][N] */
 #define TG_OP_CURL 1  /* i k x field (vorticity)   -> phys [3][N][N][N] */
 #define TG_OP_DIV 2   /* i k . field (divergence)  -> phys [N][N][N]    */
+#define TG_OP_GRAD 3  /* i k f of ONE scalar component (spec holds a single
+                         [K][K][KZ] component) -> phys [3][N][N][N]: the
+                         x, y and z derivatives                          */
 
-/* Compact spectrum (3 components) -> physical real field(s): ifftn(...).real
- * of the Hermitian full-cube spectrum of the field, its curl or divergence.
+/* Compact spectrum (3 components; 1 for TG_OP_GRAD) -> physical real
+ * field(s): ifftn(...).real of the Hermitian full-cube spectrum of the
+ * field, its curl, its divergence or the gradient of a scalar.
  * Replaces ifftn(...).real (jax/taylor_green_jax.py:185),
  * SpectralGrid.to_real (cuda/taylor_green_cuda/grid.py:36-37), vorticity
  * (jax/taylor_green_jax.py:99-109; MHD spectral_curl operators.py:24-25) and
  * spectral_divergence (jax/taylor_green_jax.py:90-96; MHD
- * divergence_from_hat operators.py:50-51). */
+ * divergence_from_hat operators.py:50-51) and spectral_gradient
+ * (jax/taylor_green_jax.py:82-87). */
 int tg_inverse(tg_plan* plan, const void* spec, int op, void* phys, void* ws, void* stream);
 
 /* Right-hand side in the compact spectrum.
@@ -106,7 +112,7 @@ int tg_rk4_steps(tg_plan* plan, void* u_hat, void* b_hat, int nsteps, double dt,
                  double eta, double lorentz_sign, void* ws, void* stream);
 
 /* Diagnostics of the compact state, FP64 compensated sums (also for FP32
- * plans).  out[0..9]:
+ * plans).  out[0..TG_DIAG_COUNT):
  *  0 E_k = 1/2 <|u|^2>                 (kinetic_energy, jax/...:199-201)
  *  1 Z   = 1/2 <|curl u|^2>            (enstrophy, jax/...:112-115)
  *  2 <|grad u|^2>  (x nu = viscous_dissipation, jax/...:118-128)
@@ -114,7 +120,11 @@ int tg_rk4_steps(tg_plan* plan, void* u_hat, void* b_hat, int nsteps, double dt,
  *  4 H_c = <u . b>                     (diagnostics.py:25-29)
  *  5 <|div u|^2>   6 <|div b|^2>   7 <|curl b|^2>
  *  8 max |div u| on the grid, 9 max |div b| (only if with_divmax, else 0)
+ * 10 number of stored real values of u_hat (and b_hat) that are NaN or
+ *    infinite: nonzero once a run has diverged (the reference propagates
+ *    NaN silently; the run loops of the Python layer raise on it)
  * Synchronises `stream`. */
+#define TG_DIAG_COUNT 11
 int tg_diagnostics(tg_plan* plan, const void* u_hat, const void* b_hat, int with_divmax,
                    double* out, void* ws, void* stream);
 

@@ -16,7 +16,8 @@ LIB_PATH = os.environ.get("TGDNS_LIB") or os.path.join(_HERE, "libtgdns.so")
 
 TG_INFO_N, TG_INFO_K, TG_INFO_KZ, TG_INFO_COMPACT_ELEMS, TG_INFO_WS_BYTES = 0, 1, 2, 3, 4
 TG_INFO_PRECISION, TG_INFO_MHD, TG_INFO_COUNT = 5, 6, 8
-TG_OP_FIELD, TG_OP_CURL, TG_OP_DIV = 0, 1, 2
+TG_OP_FIELD, TG_OP_CURL, TG_OP_DIV, TG_OP_GRAD = 0, 1, 2, 3
+TG_DIAG_COUNT = 11
 
 _lib = None
 
@@ -186,8 +187,14 @@ class Plan:
 
     def inverse(self, spec, out=None, op=TG_OP_FIELD):
         """Compact spectrum (3,K,K,KZ) -> physical real field: the field itself
-        (3,N,N,N), its curl (3,N,N,N) or its divergence (N,N,N)."""
-        self._spec(spec, "spec")
+        (3,N,N,N), its curl (3,N,N,N) or its divergence (N,N,N); for
+        TG_OP_GRAD, one scalar component (K,K,KZ) -> its gradient (3,N,N,N)."""
+        if op == TG_OP_GRAD:
+            if spec.dtype != self.cdtype or tuple(spec.shape) != self.spec_shape[1:] or not spec.is_contiguous() \
+                    or spec.device != self.device:
+                raise ValueError(f"spec: expected contiguous {self.cdtype} {self.spec_shape[1:]} on {self.device}")
+        else:
+            self._spec(spec, "spec")
         if out is None:
             out = self.empty_phys() if op != TG_OP_DIV else torch.empty(
                 self.phys_shape[1:], dtype=self.dtype, device=self.device)
@@ -289,10 +296,10 @@ class Plan:
         self._spec(u_hat, "u_hat")
         if b_hat is not None:
             self._spec(b_hat, "b_hat")
-        out = (ctypes.c_double * 10)()
+        out = (ctypes.c_double * TG_DIAG_COUNT)()
         with torch.cuda.device(self.device):
             _check(self._lib.tg_diagnostics(self._h, _ptr(u_hat), _ptr(b_hat), int(bool(divmax)), out,
                                             _ptr(self.ws), _stream()))
         keys = ("kinetic_energy", "enstrophy", "grad_sq", "magnetic_energy", "cross_helicity",
-                "div_u_sq", "div_b_sq", "current_sq", "div_u_max", "div_b_max")
+                "div_u_sq", "div_b_sq", "current_sq", "div_u_max", "div_b_max", "nonfinite")
         return {k: float(out[i]) for i, k in enumerate(keys)}

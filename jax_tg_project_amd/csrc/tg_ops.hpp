@@ -86,6 +86,7 @@ struct Launch {
         case XM_ADJQ: hipLaunchKernelGGL((k_inv_x<R, N, XM_ADJQ, V>), grid, blk, 0, s, su, sb, out, tw, ks, al, cpz); break;
         case XM_HYDRO5: hipLaunchKernelGGL((k_inv_x<R, N, XM_HYDRO5, V>), grid, blk, 0, s, su, sb, out, tw, ks, al, cpz); break;
         case XM_MHD5: hipLaunchKernelGGL((k_inv_x<R, N, XM_MHD5, V>), grid, blk, 0, s, su, sb, out, tw, ks, al, cpz); break;
+        case XM_GRAD3: hipLaunchKernelGGL((k_inv_x<R, N, XM_GRAD3, V>), grid, blk, 0, s, su, sb, out, tw, ks, al, cpz); break;
         default: hipLaunchKernelGGL((k_inv_x<R, N, XM_DIV, V>), grid, blk, 0, s, su, sb, out, tw, ks, al, cpz); break;
       }
       TG_LAUNCHED();
@@ -311,7 +312,7 @@ struct Ops {
     using Lc = Launch<R, N>;
     Ptrs<R> w = carve<R>(p, ws);
     const cpx<R>* tw = static_cast<const cpx<R>*>(p->tw);
-    const int xm = op == TG_OP_CURL ? XM_CURL3 : op == TG_OP_DIV ? XM_DIV : XM_COPY3;
+    const int xm = op == TG_OP_CURL ? XM_CURL3 : op == TG_OP_DIV ? XM_DIV : op == TG_OP_GRAD ? XM_GRAD3 : XM_COPY3;
     const int nf = op == TG_OP_DIV ? 1 : 3;
     TG_RET(Lc::inv_x(xm, 0, spec, nullptr, w.A, tw, R(p->ks), s));
     TG_RET(Lc::y(+1, 0, nf, w.A, w.B, tw, s));
@@ -622,8 +623,9 @@ struct Ops {
     out[5] = host[5] / n6;
     out[6] = host[6] / n6;
     out[7] = host[7] / n6;
-    out[8] = host[8];
-    out[9] = host[9];
+    out[8] = host[NDIAG];      // max |div u|
+    out[9] = host[NDIAG + 1];  // max |div b|
+    out[10] = host[8];         // non-finite stored values
     return TG_OK;
   }
 };

@@ -83,13 +83,13 @@ struct CfgXF {
 };
 
 enum XMode { XM_HYDRO = 0, XM_MHD = 1, XM_COPY3 = 2, XM_DIV = 3, XM_CURL3 = 4, XM_ADJ = 5, XM_ADJQ = 6,
-             XM_HYDRO5 = 7, XM_MHD5 = 8 };
+             XM_HYDRO5 = 7, XM_MHD5 = 8, XM_GRAD3 = 9 };
 enum ZMode { ZM_HYDRO = 0, ZM_MHD = 1, ZM_PHYS_OUT3 = 2, ZM_MAXABS = 3, ZM_PHYS_IN3 = 4, ZM_PHYS_OUT1 = 5, ZM_ADJ = 6 };
 enum FMode { FM_HYDRO_STAGE = 0, FM_MHD_STAGE = 1, FM_STORE3 = 2, FM_HYDRO_RHS = 3, FM_MHD_RHS = 4, FM_ADJ = 5 };
 
 __host__ __device__ constexpr int xmode_fields(int m) {
   return m == XM_HYDRO ? 6 : m == XM_MHD ? 12 : m == XM_ADJ ? 9 : m == XM_HYDRO5 ? 5 : m == XM_MHD5 ? 10
-       : (m == XM_COPY3 || m == XM_CURL3 || m == XM_ADJQ) ? 3 : 1;
+       : (m == XM_COPY3 || m == XM_CURL3 || m == XM_ADJQ || m == XM_GRAD3) ? 3 : 1;
 }
 
 // ----------------------------------------------------------------- pass 1
@@ -98,7 +98,8 @@ __host__ __device__ constexpr int xmode_fields(int m) {
 // CURL3 -> i k x S; DIV -> i k . S; ADJ -> (u, omega, P(alpha a)) of the
 // state and the cotangent; ADJQ -> P(alpha S) of a cotangent S alone;
 // HYDRO5 -> (u, i kx u_y, i kx u_z), the five fields from which the y pass
-// completes omega (k_y_curl); MHD5 -> the same for u and for b.  The 1/N^3
+// completes omega (k_y_curl); MHD5 -> the same for u and for b; GRAD3 ->
+// i k S_0, the gradient of the scalar field held in component 0.  The 1/N^3
 // inverse normalisation is applied here.
 template <typename R>
 __device__ __forceinline__ cpx<R> xfield5(int f, cpx<R> a, cpx<R> b, cpx<R> c, R kx) {
@@ -117,6 +118,11 @@ __device__ __forceinline__ cpx<R> xfield(int f, const cpx<R>* __restrict__ su,
     const int g = f >= 5 ? f - 5 : f;
     if (g < 3) return s[g * cs + pt];
     return xfield5<R>(g, cpx<R>{}, s[cs + pt], s[2 * cs + pt], kx);
+  }
+  if constexpr (MODE == XM_GRAD3) {
+    const cpx<R> a = su[pt];
+    const R k = f == 0 ? kx : (f == 1 ? ky : kz);
+    return cmuli(cpx<R>{k * a.x, k * a.y});
   }
   if constexpr (MODE == XM_ADJQ) {  // P(alpha * a) of the cotangent su
     sb = su;
@@ -158,6 +164,11 @@ __device__ __forceinline__ cpx<R> xfield(int f, const cpx<R>* __restrict__ su,
 template <typename R, int N, int MODE, int NS, int E>
 __device__ __forceinline__ cpx<R> xfield_reg(int f, const cpx<R> (&sv)[NS][E], int slot, R kx, R ky, R kz,
                                              R alpha) {
+  if constexpr (MODE == XM_GRAD3) {
+    const cpx<R> a = sv[0][slot];
+    const R k = f == 0 ? kx : (f == 1 ? ky : kz);
+    return cmuli(cpx<R>{k * a.x, k * a.y});
+  } else {
   int o = 0;
   if constexpr (MODE == XM_HYDRO5 || MODE == XM_MHD5) {
     if (f >= 5) { o = 3; f -= 5; }
@@ -193,6 +204,7 @@ __device__ __forceinline__ cpx<R> xfield_reg(int f, const cpx<R> (&sv)[NS][E], i
     else r = {kx * b.x - ky * a.x, kx * b.y - ky * a.y};
     return cmuli(r);
   }
+  }
 }
 
 template <typename R, int N, int MODE, int V>
@@ -216,7 +228,8 @@ struct XInvBody {
     stage_twiddles<R, N>(tws, tw);
     // The first FFT stage of every field reads the same E points per thread:
     // load the state (and cotangent) components at those points once.
-    constexpr int NS = (MODE == XM_HYDRO || MODE == XM_CURL3 || MODE == XM_DIV || MODE == XM_COPY3) ? 3
+    constexpr int NS = (MODE == XM_GRAD3) ? 1
+                     : (MODE == XM_HYDRO || MODE == XM_CURL3 || MODE == XM_DIV || MODE == XM_COPY3) ? 3
                      : (MODE == XM_MHD || MODE == XM_MHD5 || MODE == XM_ADJ) ? 6 : 3;
     constexpr int TPL = N / E;
     const int tid = threadIdx.x;
@@ -1687,7 +1700,9 @@ __global__ __launch_bounds__(256) void k_adj_final(const cpx<R>* __restrict__ la
 // otherwise), accumulated in FP64 with Neumaier compensation.  Slots:
 //  0 sum|u|^2  1 sum|k x u|^2  2 sum k^2|u|^2  3 sum|b|^2  4 sum Re(u.conj b)
 //  5 sum|k.u|^2  6 sum|k.b|^2  7 sum|k x b|^2
-constexpr int NDIAG = 8;
+//  8 number of stored real numbers (re and im parts of u, b) that are NaN
+//    or infinite: the failure flag of a diverging run
+constexpr int NDIAG = 9;
 constexpr int DIAG_NT = 256;
 
 struct Neu {
@@ -1718,6 +1733,9 @@ __global__ __launch_bounds__(DIAG_NT) void k_diag(const cpx<R>* __restrict__ su,
     double ur[3], ui[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) { ur[c] = double(su[c * cs + pt].x); ui[c] = double(su[c * cs + pt].y); }
+    int bad = 0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) bad += int(!isfinite(ur[c])) + int(!isfinite(ui[c]));
     const double uu = ur[0] * ur[0] + ui[0] * ui[0] + ur[1] * ur[1] + ui[1] * ui[1] + ur[2] * ur[2] + ui[2] * ui[2];
     auto curl2 = [&](const double* r, const double* i) {
       const double ar = ky * r[2] - kz * r[1], ai = ky * i[2] - kz * i[1];
@@ -1741,7 +1759,10 @@ __global__ __launch_bounds__(DIAG_NT) void k_diag(const cpx<R>* __restrict__ su,
       acc[4].add(w * (ur[0] * br[0] + ui[0] * bi[0] + ur[1] * br[1] + ui[1] * bi[1] + ur[2] * br[2] + ui[2] * bi[2]));
       acc[6].add(w * div2(br, bi));
       acc[7].add(w * curl2(br, bi));
+#pragma unroll
+      for (int c = 0; c < 3; ++c) bad += int(!isfinite(br[c])) + int(!isfinite(bi[c]));
     }
+    if (bad) acc[8].add(double(bad));
   }
   __shared__ double red[DIAG_NT];
   for (int q = 0; q < NDIAG; ++q) {

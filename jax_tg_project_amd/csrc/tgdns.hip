@@ -215,8 +215,8 @@ int tg_forward(tg_plan* p, const void* phys, void* spec, void* ws, void* stream)
 int tg_inverse(tg_plan* p, const void* spec, int op, void* phys, void* ws, void* stream) {
   TG_RET(check_plan(p));
   if (!phys || !spec || !ws) return fail(TG_ERR_ARG, "tg_inverse: NULL buffer");
-  if (op != TG_OP_FIELD && op != TG_OP_CURL && op != TG_OP_DIV)
-    return fail(TG_ERR_ARG, "tg_inverse: op must be TG_OP_FIELD, TG_OP_CURL or TG_OP_DIV");
+  if (op != TG_OP_FIELD && op != TG_OP_CURL && op != TG_OP_DIV && op != TG_OP_GRAD)
+    return fail(TG_ERR_ARG, "tg_inverse: op must be TG_OP_FIELD, TG_OP_CURL, TG_OP_DIV or TG_OP_GRAD");
   return with_table(p, [&](const auto& T, auto* rt) {
     using R = std::remove_pointer_t<decltype(rt)>;
     return T.inverse(p, static_cast<const cpx<R>*>(spec), op, static_cast<R*>(phys), ws,

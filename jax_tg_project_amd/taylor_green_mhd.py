@@ -23,6 +23,7 @@ import torch
 
 from . import _lib
 from .solver import MhdSolver, get_plan
+from .taylor_green_cuda import SpectralGrid, check_finite, rk4_linear
 
 _TWO_PI = 2.0 * math.pi
 
@@ -106,101 +107,6 @@ def _dev(device):
     return torch.device("cuda" if torch.cuda.is_available() else "cpu")
 
 
-class SpectralGrid:
-    """cuda/taylor_green_cuda/grid.py:4-40 on torch tensors; to_spectral /
-    to_real run the pruned HIP transforms."""
-
-    def __init__(self, n, domain_length, device=None):
-        self.n = int(n)
-        self.domain_length = domain_length
-        k = 2.0 * np.pi * np.fft.fftfreq(n, d=domain_length / n)
-        cutoff = (n // 3) * (2.0 * np.pi / domain_length)
-        keep = np.abs(k) <= cutoff
-        dev = _dev(device)
-        kt = torch.tensor(k, dtype=torch.float64, device=dev)
-        kx, ky, kz = torch.meshgrid(kt, kt, kt, indexing="ij")
-        kp = torch.tensor(keep, device=dev)
-        self._set_wavenumbers(kx, ky, kz, kp[:, None, None] & kp[None, :, None] & kp[None, None, :])
-
-    @classmethod
-    def from_arrays(cls, kx, ky, kz, dealias_mask):
-        from .taylor_green_jax import grid_from_arrays
-        n, L = grid_from_arrays(kx, ky, kz, dealias_mask)
-        grid = cls.__new__(cls)
-        grid.n = n
-        grid.domain_length = L
-        grid._set_wavenumbers(torch.as_tensor(kx), torch.as_tensor(ky), torch.as_tensor(kz),
-                              torch.as_tensor(dealias_mask))
-        return grid
-
-    def _set_wavenumbers(self, kx, ky, kz, dealias_mask):
-        self.kx, self.ky, self.kz = kx, ky, kz
-        self.k_squared = kx * kx + ky * ky + kz * kz
-        self.dealias_mask = dealias_mask
-
-    @property
-    def length(self):
-        return self.domain_length if self.domain_length is not None else _TWO_PI
-
-    def plan(self, dtype=torch.float64):
-        _lib._require_gpu()
-        return get_plan(self.n, dtype, False, self.length)
-
-    # compact <-> full-cube conversions (edge plumbing)
-    def _kept(self, device):
-        n, M = self.n, self.n // 3
-        idx = torch.cat([torch.arange(0, M + 1), torch.arange(n - M, n)]).to(device)
-        return idx
-
-    def full_to_compact(self, f_hat):
-        idx = self._kept(f_hat.device)
-        M = self.n // 3
-        c = f_hat.index_select(-3, idx).index_select(-2, idx)[..., : M + 1]
-        return c.contiguous()
-
-    def compact_to_full(self, c_hat):
-        n, M = self.n, self.n // 3
-        idx = self._kept(c_hat.device)
-        full = torch.zeros(c_hat.shape[:-3] + (n, n, n), dtype=c_hat.dtype, device=c_hat.device)
-        ix = idx[:, None, None]
-        iy = idx[None, :, None]
-        iz = torch.arange(M + 1, device=c_hat.device)[None, None, :]
-        full[..., ix, iy, iz] = c_hat
-        nx = (-idx) % n
-        izn = (-torch.arange(1, M + 1, device=c_hat.device)) % n
-        full[..., nx[:, None, None], nx[None, :, None], izn[None, None, :]] = torch.conj(c_hat[..., 1:])
-        return full
-
-    def to_spectral(self, field):
-        """mask * fftn(field) as a full cube (grid.py:31-34)."""
-        f = torch.as_tensor(field)
-        dtype = torch.float64 if f.dtype not in (torch.float32,) else torch.float32
-        p = self.plan(torch.float64)
-        f = f.to(p.device, torch.float64).contiguous()
-        return self.compact_to_full(p.forward(f))
-
-    def to_real(self, field_hat):
-        """ifftn(field_hat).real (grid.py:36-37) for a masked spectrum."""
-        p = self.plan(torch.float64)
-        fh = torch.as_tensor(field_hat).to(p.device, torch.complex128)
-        c = self.full_to_compact(fh)
-        _check_masked(fh, c)
-        return p.inverse(c)
-
-    def dealias(self, field_hat):
-        return field_hat * self.dealias_mask.to(field_hat.device)
-
-
-def _check_masked(full, compact):
-    """The full-cube spectrum must have nothing outside the kept modes."""
-    tot = float((full.abs() ** 2).sum())
-    kz0 = compact[..., 0].abs() ** 2
-    kept = float(kz0.sum() + 2.0 * (compact[..., 1:].abs() ** 2).sum())
-    if abs(tot - kept) > 1e-20 + 1e-9 * tot:
-        raise ValueError("spectrum has modes outside the 2/3 dealiasing mask (or is not Hermitian); "
-                         "apply grid.dealias / pass a real field's spectrum")
-
-
 def validate_timestep(dt, viscosity, resistivity, grid, max_velocity=1.0, max_field=0.5, cfl=0.5):
     """cuda/taylor_green_mhd/config.py:75-118."""
     if dt <= 0.0:
@@ -261,7 +167,11 @@ class TaylorGreenMhdInitialCondition:
 
 # ------------------------------------------------------------------ operator / integrator
 class MhdSpectralOperator:
-    """cuda/taylor_green_mhd/operators.py:4-107 on full-cube spectra."""
+    """cuda/taylor_green_mhd/operators.py:4-107 on full-cube spectra.  rhs
+    runs the fused MHD passes on the Hermitian kept part of (u_hat, b_hat);
+    a remainder (anti-Hermitian or outside the mask; zero for the spectra
+    of real fields) enters only the viscous / resistive terms, exactly as
+    in the reference (see taylor_green_cuda.SpectralGrid.split)."""
 
     def __init__(self, grid, lorentz_sign=-1.0):
         self.grid = grid
@@ -271,44 +181,87 @@ class MhdSpectralOperator:
         _lib._require_gpu()
         return get_plan(self.grid.n, torch.float64, True, self.grid.length)
 
+    def gradients(self, field_hat):
+        """operators.py:8-14: dealiased i k_d field_hat[c] (elementwise)."""
+        g = self.grid
+        waves = (g.kx, g.ky, g.kz)
+        return tuple(tuple(g.dealias(1j * g._on(w, field_hat.device) * field_hat[c]) for w in waves)
+                     for c in range(3))
+
     def rhs(self, velocity_hat, magnetic_field_hat, viscosity, resistivity):
         p = self._mhd_plan()
         g = self.grid
-        uh = g.full_to_compact(torch.as_tensor(velocity_hat).to(p.device, torch.complex128))
-        bh = g.full_to_compact(torch.as_tensor(magnetic_field_hat).to(p.device, torch.complex128))
+        uf = torch.as_tensor(velocity_hat).to(p.device, torch.complex128)
+        bf = torch.as_tensor(magnetic_field_hat).to(p.device, torch.complex128)
+        uh, ur = g.split(uf)
+        bh, br = g.split(bf)
         ru, rb = p.rhs(uh, bh, nu=viscosity, eta=resistivity, lorentz_sign=self.lorentz_sign)
-        return g.compact_to_full(ru), g.compact_to_full(rb)
+        ru, rb = g.compact_to_full(ru), g.compact_to_full(rb)
+        if ur is not None:
+            ru = ru - viscosity * g.k2_like(ur) * g.mask_like(ur) * ur
+        if br is not None:
+            rb = rb - resistivity * g.k2_like(br) * g.mask_like(br) * br
+        return ru, rb
 
     def spectral_curl_hat(self, field_hat):
-        g = self.grid
-        return g.dealias(torch.stack((1j * (g.ky * field_hat[2] - g.kz * field_hat[1]),
-                                      1j * (g.kz * field_hat[0] - g.kx * field_hat[2]),
-                                      1j * (g.kx * field_hat[1] - g.ky * field_hat[0]))))
+        """operators.py:16-22 (elementwise)."""
+        gr = self.gradients(field_hat)
+        return self.grid.dealias(torch.stack((gr[2][1] - gr[1][2], gr[0][2] - gr[2][0], gr[1][0] - gr[0][1])))
 
     def spectral_curl(self, field_hat):
+        """operators.py:24-25: to_real(spectral_curl_hat(field_hat))."""
         p = self.grid.plan(torch.float64)
-        c = self.grid.full_to_compact(torch.as_tensor(field_hat).to(p.device, torch.complex128))
+        fh = torch.as_tensor(field_hat).to(p.device, torch.complex128)
+        c, rest = self.grid.split(self.grid.dealias(fh))
         return p.inverse(c, op=_lib.TG_OP_CURL)
 
     def leray_project(self, vector_hat):
+        """operators.py:27-38 (elementwise)."""
         g = self.grid
-        k2 = torch.where(g.k_squared == 0.0, torch.ones_like(g.k_squared), g.k_squared)
-        kd = g.kx * vector_hat[0] + g.ky * vector_hat[1] + g.kz * vector_hat[2]
-        return vector_hat - torch.stack((g.kx * kd / k2, g.ky * kd / k2, g.kz * kd / k2))
+        dev = vector_hat.device
+        kx, ky, kz, k2 = (g._on(t, dev) for t in (g.kx, g.ky, g.kz, g.k_squared))
+        k2s = torch.where(k2 == 0.0, torch.ones_like(k2), k2)
+        kd = kx * vector_hat[0] + ky * vector_hat[1] + kz * vector_hat[2]
+        return vector_hat - torch.stack((kx * kd / k2s, ky * kd / k2s, kz * kd / k2s))
 
     def divergence_hat(self, vector_hat):
+        """operators.py:40-45 (elementwise)."""
         g = self.grid
-        return 1j * (g.kx * vector_hat[0] + g.ky * vector_hat[1] + g.kz * vector_hat[2])
+        dev = vector_hat.device
+        return 1j * (g._on(g.kx, dev) * vector_hat[0] + g._on(g.ky, dev) * vector_hat[1]
+                     + g._on(g.kz, dev) * vector_hat[2])
+
+    def divergence_clean(self, vector_hat):
+        """operators.py:47-48."""
+        return self.leray_project(vector_hat)
 
     def divergence_from_hat(self, vector_hat):
+        """operators.py:50-51: to_real(i k . vector_hat) for a spectrum inside the mask."""
         p = self.grid.plan(torch.float64)
-        c = self.grid.full_to_compact(torch.as_tensor(vector_hat).to(p.device, torch.complex128))
+        fh = torch.as_tensor(vector_hat).to(p.device, torch.complex128)
+        c, rest = self.grid.split(fh)
+        self.grid.require_in_mask(fh, rest)
+        return p.inverse(c, op=_lib.TG_OP_DIV)
+
+    def spectral_divergence(self, vector):
+        """operators.py:53-55: divergence of a physical vector field via its
+        unmasked spectrum, for band-limited fields (ValueError otherwise:
+        the pruned transforms hold only the kept modes)."""
+        from .taylor_green_jax import _require_band_limited
+        v = torch.as_tensor(vector)
+        dtype = torch.float32 if v.dtype == torch.float32 else torch.float64
+        p = self.grid.plan(dtype)
+        v = v.to(p.device, dtype).contiguous()
+        c = p.forward(v)
+        _require_band_limited(p, v, c)
         return p.inverse(c, op=_lib.TG_OP_DIV)
 
 
 class MhdRK4Integrator:
     """cuda/taylor_green_mhd/integrator.py:1-32: one classic RK4 step of
-    (u_hat, b_hat), all four stages in the fused GPU passes."""
+    (u_hat, b_hat), all four stages in the fused GPU passes; the result is
+    not dealiased, as there (a remainder is carried as the reference
+    carries it)."""
 
     def __init__(self, operator, dt, viscosity, resistivity):
         self.operator = operator
@@ -316,15 +269,26 @@ class MhdRK4Integrator:
         self.viscosity = viscosity
         self.resistivity = resistivity
 
-    def step(self, state):
+    def _step(self, state, dealias):
         op = self.operator
         p = op._mhd_plan()
         g = op.grid
-        uh = g.full_to_compact(torch.as_tensor(state[0]).to(p.device, torch.complex128))
-        bh = g.full_to_compact(torch.as_tensor(state[1]).to(p.device, torch.complex128))
+        uf = torch.as_tensor(state[0]).to(p.device, torch.complex128)
+        bf = torch.as_tensor(state[1]).to(p.device, torch.complex128)
+        uh, ur = g.split(uf)
+        bh, br = g.split(bf)
         p.rk4(uh, bh, nsteps=1, dt=self.dt, nu=self.viscosity, eta=self.resistivity,
               lorentz_sign=op.lorentz_sign)
-        return g.compact_to_full(uh), g.compact_to_full(bh)
+        out = [g.compact_to_full(uh), g.compact_to_full(bh)]
+        for i, (rest, coef) in enumerate(((ur, self.viscosity), (br, self.resistivity))):
+            if rest is not None:
+                m = g.mask_like(rest)
+                kept = rk4_linear(m * rest, -coef * g.k2_like(rest) * m, self.dt)
+                out[i] = out[i] + (kept if dealias else kept + (1.0 - m) * rest)
+        return out[0], out[1]
+
+    def step(self, state):
+        return self._step(state, dealias=False)
 
 
 class MhdDiagnostics:
@@ -356,8 +320,12 @@ class MhdDiagnostics:
     def record(self, time_value, velocity_hat, magnetic_field_hat):
         g = self.operator.grid
         p = get_plan(g.n, torch.float64, True, g.length)
-        uh = g.full_to_compact(torch.as_tensor(velocity_hat).to(p.device, torch.complex128))
-        bh = g.full_to_compact(torch.as_tensor(magnetic_field_hat).to(p.device, torch.complex128))
+        uf = torch.as_tensor(velocity_hat).to(p.device, torch.complex128)
+        bf = torch.as_tensor(magnetic_field_hat).to(p.device, torch.complex128)
+        uh, ur = g.split(uf)
+        bh, br = g.split(bf)
+        g.require_in_mask(uf, ur)
+        g.require_in_mask(bf, br)
         return self.record_values(time_value, p.diagnostics(uh, bh))
 
 
@@ -365,7 +333,8 @@ class TaylorGreenMhdSimulation:
     """cuda/taylor_green_mhd/simulation.py:15-100; the state stays compact on
     the GPU between saves and the steps between saves run as one call."""
 
-    def __init__(self, config=None, make_plots=True):
+    def __init__(self, config=None, make_plots=True, check_finite=True):
+        self.check_finite = check_finite
         self.config = config or TaylorGreenMhdConfig()
         c = self.config
         _lib._require_gpu()
@@ -390,8 +359,14 @@ class TaylorGreenMhdSimulation:
     def magnetic_field_hat(self):
         return self.grid.compact_to_full(self.solver.b_hat.to(torch.complex128))
 
+    def _diag(self, step):
+        d = self.solver.diagnostics()
+        if self.check_finite:
+            check_finite(d, step, step * self.config.dt)
+        return d
+
     def record(self, step):
-        k, m, h, z = self.diagnostics.record_values(step * self.config.dt, self.solver.diagnostics())
+        k, m, h, z = self.diagnostics.record_values(step * self.config.dt, self._diag(step))
         print(f"Save time step: {step}, time: {step * self.config.dt:.6f}, "
               f"E_k: {k:.8e}, E_m: {m:.8e}, H_c: {h:.8e}, enstrophy: {z:.8e}")
         return k, m, h, z
@@ -400,7 +375,7 @@ class TaylorGreenMhdSimulation:
         c = self.config
         max_field = c.magnetic_amplitude if c.initial_condition == "reference" else 2.0
         validate_timestep(c.dt, c.viscosity, c.resistivity, self.grid, max_velocity=1.0, max_field=max_field)
-        self.diagnostics.record_values(0.0, self.solver.diagnostics())
+        self.diagnostics.record_values(0.0, self._diag(0))
         start = time.perf_counter()
         step = 0
         while step < c.steps:
@@ -424,38 +399,43 @@ class TaylorGreenMhdSimulation:
 
 
 def _save_plots(sim):
-    """cuda/taylor_green_mhd/plotting.py:9-47 (PNG side effect)."""
+    """cuda/taylor_green_mhd/plotting.py:9-47 (PNG side effect, same files,
+    panels and labels)."""
     try:
         import matplotlib
         matplotlib.use("Agg")
         import matplotlib.pyplot as plt
     except ImportError:  # pragma: no cover
+        print("matplotlib unavailable: plots skipped")
         return
     n = sim.config.n
     u, b = sim.solver.fields()
-    us, bs = u[:, :, :, n // 2].cpu().numpy(), b[:, :, :, n // 2].cpu().numpy()
-    fig, axes = plt.subplots(2, 3, figsize=(12, 8))
-    for i in range(3):
-        for row, (f, lab) in enumerate(((us, "u"), (bs, "B"))):
-            im = axes[row, i].imshow(f[i].T, origin="lower", cmap="viridis")
-            axes[row, i].set_title(f"{lab}{i + 1} slice")
-            fig.colorbar(im, ax=axes[row, i])
-    plt.tight_layout()
-    plt.savefig("taylor_green_mhd_slice.png", dpi=150)
-    plt.close(fig)
+    velocity_slice = u[:, :, :, n // 2].double().cpu().numpy()
+    magnetic_slice = b[:, :, :, n // 2].double().cpu().numpy()
+    figure, axes = plt.subplots(2, 3, figsize=(14, 8))
+    for component in range(3):
+        image = axes[0, component].imshow(velocity_slice[component].T, origin="lower", cmap="viridis")
+        axes[0, component].set_title(f"u{component + 1} slice")
+        figure.colorbar(image, ax=axes[0, component])
+        image = axes[1, component].imshow(magnetic_slice[component].T, origin="lower", cmap="plasma")
+        axes[1, component].set_title(f"B{component + 1} slice")
+        figure.colorbar(image, ax=axes[1, component])
+    figure.tight_layout()
+    figure.savefig("taylor_green_mhd_slice.png", dpi=200)
+    plt.close(figure)
     d = sim.diagnostics
-    fig, (a1, a2) = plt.subplots(2, 1, figsize=(10, 8))
-    a1.plot(d.times, d.kinetic_energies, label="E_k")
-    a1.plot(d.times, d.magnetic_energies, label="E_m")
-    a1.plot(d.times, d.total_energies, label="E_total")
-    a1.legend()
-    a1.grid(True, alpha=0.3)
-    a2.plot(d.times, d.cross_helicities, label="H_c")
-    a2.legend()
-    a2.grid(True, alpha=0.3)
-    plt.tight_layout()
-    plt.savefig("taylor_green_mhd_diagnostics.png", dpi=150)
-    plt.close(fig)
+    figure, axes = plt.subplots(2, 2, figsize=(12, 10))
+    panels = ((0, 0, d.kinetic_energies, "b-o", "Kinetic energy"), (0, 1, d.magnetic_energies, "r-o", "Magnetic energy"),
+              (1, 0, d.total_energies, "g-o", "Total energy"), (1, 1, d.cross_helicities, "m-s", "Cross helicity"))
+    for r, c, values, style, label in panels:
+        axes[r, c].plot(d.times, values, style, markersize=4)
+        axes[r, c].set_ylabel(label)
+        axes[r, c].grid(True, alpha=0.3)
+        if r == 1:
+            axes[r, c].set_xlabel("Time")
+    figure.tight_layout()
+    figure.savefig("taylor_green_mhd_diagnostics.png", dpi=200)
+    plt.close(figure)
     print("Saved: taylor_green_mhd_slice.png")
     print("Saved: taylor_green_mhd_diagnostics.png")
 
@@ -496,7 +476,7 @@ def advance_one_step(velocity_hat, magnetic_field_hat, dt, viscosity, resistivit
     """cuda/taylor_green_mhd.py:49-59 (the result is dealiased, as there)."""
     grid = SpectralGrid.from_arrays(kx, ky, kz, dealias_mask)
     integ = MhdRK4Integrator(MhdSpectralOperator(grid, lorentz_sign), dt, viscosity, resistivity)
-    return integ.step((velocity_hat, magnetic_field_hat))
+    return integ._step((velocity_hat, magnetic_field_hat), dealias=True)
 
 
 def kinetic_energy(velocity):

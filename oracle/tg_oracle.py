@@ -255,6 +255,66 @@ class SpectralGrid:
         return field_hat * self.dealias_mask
 
 
+def cupy_rhs(velocity_hat, viscosity, grid):
+    """cuda/taylor_green_cuda/operators.py:8-48: on the full-cube spectrum,
+    dealias, u = ifftn(u_hat).real, gradients i k_d u_hat_c dealiased,
+    n_c = sum_d u_d d_d u_c (the product of the DEALIASED velocity, unlike
+    the JAX path's raw u), mask F(n), Leray projection with k2 == 0 -> 1,
+    -P n_hat - nu k^2 u_hat."""
+    kx, ky, kz = grid.kx, grid.ky, grid.kz
+    u_hat = grid.dealias(velocity_hat)
+    u = grid.to_real(u_hat)
+    waves = (kx, ky, kz)
+    nl = np.stack([sum(u[d] * grid.to_real(grid.dealias(1j * waves[d] * u_hat[c])) for d in range(3))
+                   for c in range(3)])
+    n_hat = _fftn(nl) * grid.dealias_mask
+    return -leray(n_hat, kx, ky, kz) - viscosity * grid.k_squared * u_hat
+
+
+def cupy_rk4_step(state, dt, viscosity, grid):
+    """cuda/taylor_green_cuda/integrator.py:7-12 (no dealias afterwards)."""
+    r1 = cupy_rhs(state, viscosity, grid)
+    r2 = cupy_rhs(state + 0.5 * dt * r1, viscosity, grid)
+    r3 = cupy_rhs(state + 0.5 * dt * r2, viscosity, grid)
+    r4 = cupy_rhs(state + dt * r3, viscosity, grid)
+    return state + (dt / 6.0) * (r1 + 2.0 * r2 + 2.0 * r3 + r4)
+
+
+def cupy_diagnostics(state_hat, grid):
+    """cuda/taylor_green_cuda/diagnostics.py:15-25 -> (E, Z): E from
+    to_real(state), Z from the dealiased spectral curl (operators.py:16-22)."""
+    u = grid.to_real(state_hat)
+    om = grid.to_real(_curl_hat(grid, state_hat))
+    return (float(0.5 * np.mean(np.sum(u * u, axis=0))), float(0.5 * np.mean(np.sum(om * om, axis=0))))
+
+
+def cupy_run(n, dt, viscosity, total_time, save_every_time, u0=None, length=2.0 * np.pi):
+    """cuda/taylor_green_cuda/simulation.py:28-52 without plots and prints:
+    record(0), then per step RK4 + dealias, record when step % save_every
+    == 0 or step == steps at time step * dt.  Returns (state, times,
+    energies, enstrophies)."""
+    steps = int(total_time / dt)
+    save_every = max(1, int(save_every_time / dt))
+    grid = SpectralGrid(n, length)
+    if u0 is None:
+        u0 = initial_taylor_green(n, length)
+    state = grid.to_spectral(np.asarray(u0, dtype=np.float64))
+    times, energies, enstrophies = [], [], []
+
+    def record(t):
+        e, z = cupy_diagnostics(state, grid)
+        times.append(t)
+        energies.append(e)
+        enstrophies.append(z)
+
+    record(0.0)
+    for step in range(1, steps + 1):
+        state = grid.dealias(cupy_rk4_step(state, dt, viscosity, grid))
+        if step % save_every == 0 or step == steps:
+            record(step * dt)
+    return state, times, energies, enstrophies
+
+
 def mhd_initial_condition(n, length=2.0 * np.pi, magnetic_amplitude=0.5, kind="reference"):
     """Physical (u, b).
 
@@ -318,9 +378,10 @@ def mhd_rhs(u_hat, b_hat, viscosity, resistivity, grid, lorentz_sign=-1.0):
     return rhs_u, rhs_b
 
 
-def mhd_rk4_step(u_hat, b_hat, dt, viscosity, resistivity, grid, lorentz_sign=-1.0):
+def mhd_rk4_step(u_hat, b_hat, dt, viscosity, resistivity, grid, lorentz_sign=-1.0, dealias=True):
     """cuda/taylor_green_mhd/integrator.py:14-32 plus the shim's final dealias
-    (cuda/taylor_green_mhd.py:58-59)."""
+    (cuda/taylor_green_mhd.py:58-59); ``dealias=False`` is the integrator's
+    step alone."""
     def f(a, b):
         return mhd_rhs(a, b, viscosity, resistivity, grid, lorentz_sign)
     k1u, k1b = f(u_hat, b_hat)
@@ -329,6 +390,8 @@ def mhd_rk4_step(u_hat, b_hat, dt, viscosity, resistivity, grid, lorentz_sign=-1
     k4u, k4b = f(u_hat + dt * k3u, b_hat + dt * k3b)
     un = u_hat + (dt / 6.0) * (k1u + 2.0 * k2u + 2.0 * k3u + k4u)
     bn = b_hat + (dt / 6.0) * (k1b + 2.0 * k2b + 2.0 * k3b + k4b)
+    if not dealias:
+        return un, bn
     return grid.dealias(un), grid.dealias(bn)
 
 
@@ -344,6 +407,46 @@ def mhd_diagnostics(u_hat, b_hat, grid):
     du = float(np.max(np.abs(grid.to_real(_div_hat(grid, u_hat)))))
     db = float(np.max(np.abs(grid.to_real(_div_hat(grid, b_hat)))))
     return float(ek), float(em), float(hc), float(z), du, db
+
+
+def mhd_run(n, dt, viscosity, resistivity, total_time, save_every_time, magnetic_amplitude=0.5,
+            kind="reference", lorentz_sign=-1.0, u0=None, b0=None, length=2.0 * np.pi, max_field=None):
+    """cuda/taylor_green_mhd/simulation.py:49-100 (TaylorGreenMhdSimulation.run
+    without plots and prints), with the step count and save cadence of
+    config.py:44-50: validate the time step, record at t = 0, then per step
+    RK4 + dealias (integrator.py:14-32, simulation.py:63-66) and record when
+    ``step % save_every == 0 or step == steps`` at time ``step * dt``.
+
+    ``u0``/``b0`` (physical) replace the initial condition so the GPU and
+    the oracle start from identical arrays; ``max_field`` defaults to the
+    magnetic amplitude as the reference passes it.  Returns (u_hat, b_hat,
+    records) with the MhdDiagnostics lists (diagnostics.py:4-49).
+    """
+    steps = int(total_time / dt)
+    save_every = max(1, int(save_every_time / dt))
+    grid = SpectralGrid(n, length)
+    if u0 is None:
+        u0, b0 = mhd_initial_condition(n, length, magnetic_amplitude, kind)
+    uh = grid.to_spectral(np.asarray(u0, dtype=np.float64))
+    bh = grid.to_spectral(np.asarray(b0, dtype=np.float64))
+    mhd_validate_timestep(dt, viscosity, resistivity, grid, max_velocity=1.0,
+                          max_field=magnetic_amplitude if max_field is None else max_field)
+    rec = {k: [] for k in ("times", "kinetic_energies", "magnetic_energies", "total_energies",
+                           "cross_helicities", "enstrophies", "divergence_u_max", "divergence_b_max")}
+
+    def record(t):
+        ek, em, hc, z, du, db = mhd_diagnostics(uh, bh, grid)
+        for k, v in zip(("times", "kinetic_energies", "magnetic_energies", "total_energies",
+                         "cross_helicities", "enstrophies", "divergence_u_max", "divergence_b_max"),
+                        (t, ek, em, ek + em, hc, z, du, db)):
+            rec[k].append(v)
+
+    record(0.0)
+    for step in range(1, steps + 1):
+        uh, bh = mhd_rk4_step(uh, bh, dt, viscosity, resistivity, grid, lorentz_sign)
+        if step % save_every == 0 or step == steps:
+            record(step * dt)
+    return uh, bh, rec
 
 
 def mhd_validate_timestep(dt, viscosity, resistivity, grid, max_velocity=1.0,

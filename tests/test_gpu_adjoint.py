@@ -210,6 +210,36 @@ def test_fp32_adjoint_vs_fp64():
     assert abs(n32 - n64) <= 1e-4 * abs(n64)
 
 
+def test_fp32_adjoint_config_128_vs_fp64():
+    """The adjoint config (N = 128, nu = 1/400, dt = 5e-3, 200 steps, J = Z(T),
+    sqrt(200) checkpointing) in FP32 against the FP64 gradient, which the
+    central-difference test above pins: J to 1e-5, the gradient and dJ/dnu to
+    1e-4 relative."""
+    from jax_tg_project_amd import ensemble
+    from jax_tg_project_amd.adjoint import sqrt_segment
+    N, nu, dt, n = 128, 1.0 / 400, 5e-3, 200
+    u0 = ensemble.perturbed_taylor_green(N, 2024)
+    seg = sqrt_segment(n)
+    res = {}
+    for dtype in (torch.float64, torch.float32):
+        p = _plan(N, dtype)
+        x = p.forward(torch.tensor(u0, dtype=dtype, device="cuda"))
+        aws = p.adjoint_workspace(n, seg)
+        J = p.adjoint_forward(x, n, dt, nu, 1, seg, aws)
+        g, dnu = p.adjoint_backward(n, dt, nu, seg, aws)
+        res[dtype] = (J, g.to(torch.complex128), dnu)
+        del aws
+        torch.cuda.empty_cache()
+    J64, g64, n64 = res[torch.float64]
+    J32, g32, n32 = res[torch.float32]
+    eg = (torch.linalg.vector_norm(g32 - g64) / torch.linalg.vector_norm(g64)).item()
+    print(f"FP32 vs FP64 at 128^3: J {abs(J32 - J64) / J64:.2e}, grad {eg:.2e}, "
+          f"dJ/dnu {abs(n32 - n64) / abs(n64):.2e}", flush=True)
+    assert abs(J32 - J64) <= 1e-5 * J64
+    assert eg <= 1e-4
+    assert abs(n32 - n64) <= 1e-4 * abs(n64)
+
+
 @pytest.mark.parametrize("N", [64, 96, 128, 256])
 def test_vjp_matches_composed_transforms(N):
     """The fused VJP against the same formula composed from the separately

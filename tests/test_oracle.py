@@ -238,3 +238,82 @@ def test_mhd_reference_unit_invariants():
     assert np.all(np.isfinite(uh)) and np.all(np.isfinite(bh))
     *_, du, db = o.mhd_diagnostics(uh, bh, g8)
     assert du < 1e-4 and db < 1e-4
+
+
+def test_mhd_reference_ic_closed_form_diagnostics():
+    """b0 = 0.5 u_TG: E_k = 1/8, E_m = 1/4 * 1/8 = 1/32, H_c = <u.b> =
+    0.5 <|u|^2> = 1/8 (no 1/2, diagnostics.py:25-29: a 1/2 factor would give
+    1/16), Z = 3/8."""
+    grid = o.SpectralGrid(16, L)
+    u0, b0 = o.mhd_initial_condition(16, kind="reference")
+    ek, em, hc, z, du, db = o.mhd_diagnostics(grid.to_spectral(u0), grid.to_spectral(b0), grid)
+    assert abs(ek - 0.125) < 1e-15 and abs(em - 0.03125) < 1e-15
+    assert abs(hc - 0.125) < 1e-15
+    assert abs(z - 0.375) < 1e-14
+    assert du < 1e-13 and db < 1e-13
+
+
+def test_mhd_config_ic_enstrophy():
+    """Z(0) = 3/8 for u_TG whatever b is (the kinetic enstrophy)."""
+    grid = o.SpectralGrid(16, L)
+    u0, b0 = o.mhd_initial_condition(16, kind="config")
+    z = o.mhd_diagnostics(grid.to_spectral(u0), grid.to_spectral(b0), grid)[3]
+    assert abs(z - 0.375) < 1e-14
+
+
+def test_mhd_run_cadence_times_and_records():
+    """The reference's short-run test (cuda/test_taylor_green_mhd.py:50-66):
+    n=8, dt=0.005, total 0.02, saves every 0.01 -> records at steps 0, 2, 4,
+    times step*dt exactly; the first record is the t=0 closed form, the last
+    the diagnostics of the returned state; a non-divisible total time records
+    the last step as well."""
+    uh, bh, rec = o.mhd_run(8, 0.005, L / 100.0, L / 100.0, 0.02, 0.01)
+    assert rec["times"] == [0.0, 2 * 0.005, 4 * 0.005]
+    assert abs(rec["kinetic_energies"][0] - 0.125) < 1e-15
+    assert abs(rec["cross_helicities"][0] - 0.125) < 1e-15
+    assert rec["total_energies"][-1] == rec["kinetic_energies"][-1] + rec["magnetic_energies"][-1]
+    grid = o.SpectralGrid(8, L)
+    last = o.mhd_diagnostics(uh, bh, grid)
+    assert last[0] == rec["kinetic_energies"][-1] and last[4] == rec["divergence_u_max"][-1]
+    assert max(rec["divergence_u_max"]) < 1e-4 and max(rec["divergence_b_max"]) < 1e-4
+    _, _, rec2 = o.mhd_run(8, 0.005, L / 100.0, L / 100.0, 0.035, 0.01)
+    assert rec2["times"] == [0.0, 2 * 0.005, 4 * 0.005, 6 * 0.005, 7 * 0.005]
+    with pytest.raises(ValueError, match="stability limit"):
+        o.mhd_run(8, 1.0, L / 100.0, L / 100.0, 2.0, 1.0)
+
+
+def test_cupy_rhs_equals_jax_rhs_on_band_limited_states_and_closed_form():
+    """The CuPy operator (operators.py:24-48) multiplies the dealiased u, the
+    JAX one the raw u (jax/taylor_green_jax.py:165): equal for band-limited
+    states, so the notebook-pinned JAX restatement pins it; plus the t=0
+    closed form -P(n) - 3 nu u0."""
+    N, nu = 16, 0.02
+    grid = o.SpectralGrid(N, L)
+    kx, ky, kz = o.make_wavenumbers(N, L)
+    m = o.make_dealias_mask(N, L)
+    u = band_limited_field(N, 7)
+    r_cupy = grid.to_real(o.cupy_rhs(grid.to_spectral(u), nu, grid))
+    r_jax = o.spectral_rhs(u, nu, kx, ky, kz, m)
+    assert rel_l2(r_cupy, r_jax) < 1e-13
+    u0 = o.initial_taylor_green(N)
+    x = np.linspace(0, L, N, endpoint=False)
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    Pn = np.stack([np.sin(2 * X) * np.cos(2 * Z), np.sin(2 * Y) * np.cos(2 * Z),
+                   -(np.cos(2 * X) + np.cos(2 * Y)) * np.sin(2 * Z)]) / 8
+    r0 = grid.to_real(o.cupy_rhs(grid.to_spectral(u0), nu, grid))
+    assert np.max(np.abs(r0 - (-Pn - 3 * nu * u0))) < 1e-14
+
+
+def test_cupy_run_matches_reference_test_and_jax_run():
+    """cuda/test_taylor_green_cuda.py:25-37: n=8, dt=0.005, Re=100,
+    total 0.02, saves every 0.01 -> 3 records, finite; and the same run as
+    the JAX restatement's E history (both advance band-limited states)."""
+    nu = L / 100.0
+    state, times, energies, enstrophies = o.cupy_run(8, 0.005, nu, 0.02, 0.01)
+    assert times == [0.0, 0.01, 0.02]
+    assert np.all(np.isfinite(state))
+    assert abs(energies[0] - 0.125) < 1e-15 and abs(enstrophies[0] - 0.375) < 1e-14
+    _, e_jax, d_jax = o.run_simulation(N=8, dt=0.005, nu=nu, total_time=0.02, save_every_time=0.01,
+                                       return_diagnostics=True)
+    assert np.allclose(energies, e_jax, rtol=1e-13, atol=0)
+    assert np.allclose(enstrophies, d_jax["enstrophy"], rtol=1e-12, atol=0)
