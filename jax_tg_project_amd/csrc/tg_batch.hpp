@@ -1,5 +1,6 @@
 // Batched ensemble: many independent hydro instances of different grid
-// sizes advance together, one launch per pass and grid size.
+// sizes advance together, one launch per pass and grid size, on the dense
+// layout of tg_ens.hpp.
 //
 // Instances are grouped by grid size.  For every pass, one launch per size
 // class covers all instances of that size: a 1D grid whose blocks are the
@@ -11,6 +12,9 @@
 #pragma once
 #include "tg_common.hpp"
 #include "tg_passes.hpp"
+#include "tg_ens.hpp"
+
+#include <algorithm>
 
 namespace tg {
 
@@ -25,11 +29,11 @@ template <typename R>
 struct EnsInst {
   int n;
   R nu, ks;
-  cpx<R>* u0;   // compact state (3 components)
+  cpx<R>* s0;   // state (3 components, dense rows, plane pitch PA): copied in and out per call
   cpx<R>* u1;   // stage input
   cpx<R>* acc;  // r1 + 2 r2 + 2 r3
-  cpx<R>* A;    // workspace (6 N K KZ)
-  cpx<R>* B;    // workspace (6 N N KZ)
+  cpx<R>* A;    // inverse-x / z / ... workspace (see EnsCfg)
+  cpx<R>* B;
   const cpx<R>* tw;
 };
 
@@ -38,82 +42,93 @@ inline __host__ __device__ constexpr int cdiv_c(int a, int b) { return (a + b - 
 // Blocks of one instance's grid for each pass.
 template <typename R, int N>
 struct EnsGrid {
-  using G = Geo<N>;
-  using XY = CfgXY<R, N, 0>;
-  using XF = CfgXF<R, N, 0>;
-  static constexpr int GX_XY = cdiv_c(G::KZ, XY::L);
+  using C = EnsCfg<R, N>;
   static constexpr int LZ = CfgZM<R, N, ZM_HYDRO, 0>::LZ;
-  static constexpr int GX_XF = cdiv_c(G::KZ, XF::LINES / 3);
   static constexpr int ZE = ZwDefault<N>::E, ZWPB = ZwDefault<N>::WPB;
   static constexpr int zlines() {
     if constexpr (ZE > 0) return CfgZW<R, N, ZE, ZWPB>::LINES;
     else return LZ;
   }
-  static constexpr int ZBLOCKS = N * N / zlines();
+  static constexpr int GROUPS = N / C::PPB;  // plane groups of one field
+  // The six output fields of a plane group go to one XCD (workgroups are
+  // dealt to the 8 XCDs round robin), so the components they share come
+  // from that XCD's L2; possible when every instance of the class has a
+  // multiple of 48 inverse-y blocks, which keeps the block offsets of the
+  // instances multiples of 8.
+  static constexpr bool XCD_GROUPED = (GROUPS * 6) % 48 == 0;
   static constexpr int blocks(int pass) {
-    return pass == EP_INV_X ? GX_XY * G::K
-         : pass == EP_INV_Y ? GX_XY * N * 6
-         : pass == EP_Z ? ZBLOCKS
-         : pass == EP_FWD_Y ? GX_XY * N * 3
-         : GX_XF * G::K;
+    return pass == EP_INV_X ? C::TX
+         : pass == EP_INV_Y ? GROUPS * 6
+         : pass == EP_Z ? N * N / zlines()
+         : pass == EP_FWD_Y ? GROUPS * 3
+         : C::TX;
   }
 };
 
 template <typename R, int N, int PASS>
 __device__ __forceinline__ void ens_run(cpx<R>* sm, int lb, const EnsInst<R>& I, R dt, int stage) {
   using Gd = EnsGrid<R, N>;
+  constexpr int KZ = Geo<N>::KZ;
+  const cpx<R>* us = stage == 0 ? I.s0 : I.u1;
   if constexpr (PASS == EP_INV_X) {
-    const cpx<R>* us = stage == 0 ? I.u0 : I.u1;
-    XInvBody<R, N, XM_HYDRO5, 0>::run(sm, lb % Gd::GX_XY, lb / Gd::GX_XY, us, nullptr, I.A, I.tw, I.ks);
+    EnsInvX<R, N>::run(sm, lb, us, I.A, I.tw, I.ks);
   } else if constexpr (PASS == EP_INV_Y) {
-    const int bx = lb % Gd::GX_XY, rest = lb / Gd::GX_XY;
-    YCurlBody<R, N, 0>::run(sm, bx, rest % N, rest / N, 0, I.A, I.B, I.tw, I.ks);
+    int g, f;
+    if constexpr (Gd::XCD_GROUPED) {
+      const int xcd = lb & 7, j = lb >> 3;
+      g = (j / 6) * 8 + xcd;
+      f = j % 6;
+    } else {
+      g = lb % Gd::GROUPS;
+      f = lb / Gd::GROUPS;
+    }
+    EnsInvY<R, N>::run(sm, g, f, I.A, I.B, I.tw, I.ks);
   } else if constexpr (PASS == EP_FWD_Y) {
-    const int bx = lb % Gd::GX_XY, rest = lb / Gd::GX_XY;
-    YBody<R, N, -1, 0>::run(sm, bx, rest % N, rest / N, I.A, I.B, I.tw);
+    EnsFwdY<R, N>::run(sm, lb % Gd::GROUPS, lb / Gd::GROUPS, I.A, I.B, I.tw);
   } else if constexpr (PASS == EP_Z) {
     if constexpr (Gd::ZE > 0) {
-      ZWBody<R, N, Gd::ZE, Gd::ZWPB>::run(sm, lb, I.B, I.A, I.tw);
+      ZWBody<R, N, Gd::ZE, Gd::ZWPB, KZ>::run(sm, lb, I.B, I.A, I.tw);
     } else {
       constexpr int gx = N / Gd::LZ;
-      ZBody<R, N, ZM_HYDRO, 0>::run(sm, lb % gx, lb / gx, I.B, I.A, nullptr, nullptr, I.tw, R(0));
+      ZBody<R, N, ZM_HYDRO, 0, KZ>::run(sm, lb % gx, lb / gx, I.B, I.A, nullptr, nullptr, I.tw, R(0));
     }
   } else {
-    StageArgs<R> a{};
-    a.u0 = I.u0;
-    a.us = stage == 0 ? I.u0 : I.u1;
-    a.unext = stage == 3 ? I.u0 : I.u1;
-    a.uacc = I.acc;
+    EnsStage<R> a;
+    a.us = us;
+    a.u0 = I.s0;
+    a.unext = stage == 3 ? I.s0 : I.u1;
+    a.acc = I.acc;
     a.dt = dt;
     a.nu = I.nu;
     a.ks = I.ks;
     a.stage = stage;
-    XFBody<R, N, FM_HYDRO_STAGE, 0>::run(sm, lb % Gd::GX_XF, lb / Gd::GX_XF, I.B, a, I.tw);
+    EnsFwdX<R, N>::run(sm, lb, I.B, a, I.tw);
   }
 }
 
 template <typename R, int N, bool ZW = (ZwDefault<N>::E > 0)>
 struct EnsZ {  // LDS-resident z body
   static constexpr int NT = CfgZM<R, N, ZM_HYDRO, 0>::NT;
-  static constexpr int SMEM = ZBody<R, N, ZM_HYDRO, 0>::SMEM;
+  static constexpr int SMEM = ZBody<R, N, ZM_HYDRO, 0, Geo<N>::KZ>::SMEM;
   static constexpr int WAVES = 1;
 };
 template <typename R, int N>
 struct EnsZ<R, N, true> {  // register-resident z body
-  static constexpr int NT = 64 * ZwDefault<N>::WPB;
-  static constexpr int SMEM = ZWBody<R, N, ZwDefault<N>::E, ZwDefault<N>::WPB>::SMEM;
+  static constexpr int NT = 64 * EnsGrid<R, N>::ZWPB;
+  static constexpr int SMEM = ZWBody<R, N, ZwDefault<N>::E, EnsGrid<R, N>::ZWPB, Geo<N>::KZ>::SMEM;
   static constexpr int WAVES = zw_waves_per_eu<R, ZwDefault<N>::E>();
 };
 
 template <typename R, int N, int PASS>
 struct EnsBlock {
+  using C = EnsCfg<R, N>;
   static constexpr int NT = PASS == EP_Z ? EnsZ<R, N>::NT
-                          : PASS == EP_FWD_X ? CfgXF<R, N, 0>::NT : CfgXY<R, N, 0>::NT;
-  static constexpr int SMEM = PASS == EP_INV_X ? XInvBody<R, N, XM_HYDRO5, 0>::SMEM
-                            : PASS == EP_INV_Y ? YCurlBody<R, N, 0>::SMEM
+                          : (PASS == EP_INV_Y || PASS == EP_FWD_Y) ? C::NTY : C::NTX;
+  static constexpr int SMEM = PASS == EP_INV_X ? EnsInvX<R, N>::SMEM
+                            : PASS == EP_INV_Y ? EnsInvY<R, N>::SMEM
                             : PASS == EP_Z ? EnsZ<R, N>::SMEM
-                            : PASS == EP_FWD_X ? XFBody<R, N, FM_HYDRO_STAGE, 0>::SMEM
-                            : YBody<R, N, 1, 0>::SMEM;
+                            : PASS == EP_FWD_X ? EnsFwdX<R, N>::SMEM
+                            : EnsFwdY<R, N>::SMEM;
   static constexpr int WAVES = PASS == EP_Z ? EnsZ<R, N>::WAVES : 1;
 };
 
@@ -150,6 +165,8 @@ template <typename R>
 struct EnsTable {
   int n;
   int blocks[EP_COUNT];
+  int pa;                    // plane pitch of the state arrays (complex elements)
+  long long a_elems, b_elems;  // workspace arrays A, B (complex elements)
   int (*stage)(const EnsInst<R>* inst, const int* dpre, const int* hpre, int cnt, int st, double dt,
                hipStream_t s);
 };
@@ -176,8 +193,12 @@ template <typename R, int N>
 EnsTable<R> make_ens_table() {
   EnsTable<R> t{};
   if constexpr (is_ens_size(N)) {
+    using C = EnsCfg<R, N>;
     t.n = N;
     for (int p = 0; p < EP_COUNT; ++p) t.blocks[p] = EnsGrid<R, N>::blocks(p);
+    t.pa = C::PA;
+    t.a_elems = std::max(5LL * N * C::PA, 3LL * N * C::PB);
+    t.b_elems = std::max(6LL * N * C::PB, 3LL * N * C::PA);
     t.stage = &ens_stage<R, N>;
   }
   return t;

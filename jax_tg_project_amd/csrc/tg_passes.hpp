@@ -489,7 +489,9 @@ __device__ __forceinline__ void cross3(const R* a, const R* b, R* o) {
   o[2] = a[0] * b[1] - a[1] * b[0];
 }
 
-template <typename R, int N, int MODE, int V>
+// PITCH: row pitch (complex elements) of the (x, y, kz) arrays it reads and
+// writes (kzp for the single-grid plans, KZ for the dense ensemble layout).
+template <typename R, int N, int MODE, int V, int PITCH = kzp<R, N>()>
 struct ZBody {
   using G = Geo<N>;
   using Z = CfgZM<R, N, MODE, V>;
@@ -517,7 +519,7 @@ struct ZBody {
   const int tid = threadIdx.x;
   const bool live = tid < NT;
   const int x = by, y0 = bx * LZ;
-  constexpr int KZP = kzp<R, N>();  // pitch of the A/B rows
+  constexpr int KZP = PITCH;  // pitch of the A/B rows
   const size_t NNKZ = size_t(N) * N * KZP;
   const size_t line0 = size_t(x) * N + y0;  // (x, y0) line index
 
@@ -960,7 +962,7 @@ __device__ __forceinline__ void zw_store_pair(const cpx<R>* x, cpx<R>* lb, cpx<R
   wave_lds_sync();
 }
 
-template <typename R, int N, int E_, int WPB>
+template <typename R, int N, int E_, int WPB, int PITCH = kzp<R, N>()>
 struct ZWBody {
   using G = Geo<N>;
   using C = CfgZW<R, N, E_, WPB>;
@@ -970,7 +972,7 @@ struct ZWBody {
   __device__ __forceinline__ static void run(cpx<R>* sm, int blk, const cpx<R>* __restrict__ in,
                                              cpx<R>* __restrict__ out, const cpx<R>* __restrict__ tw) {
     constexpr int E = C::E, TPL = C::TPL, LW = C::LW, PS = C::PS, KZ = G::KZ;
-    constexpr int KZP = kzp<R, N>();
+    constexpr int KZP = PITCH;
     constexpr int R1 = FirstRadix<IP>::value, NB1 = E / R1, S1 = N / R1;
     const size_t NNKZ = size_t(N) * N * KZP;
     cpx<R>* tsi = sm;

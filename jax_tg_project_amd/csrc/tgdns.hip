@@ -361,8 +361,9 @@ struct tg_batch {
   std::vector<double> nu;
   std::vector<long long> state_off;  // complex elements, ninst + 1 entries
   std::vector<size_t> ws_off;        // bytes, ninst + 1 entries
+  std::vector<void*> dense, padded;  // per instance: caller's state, working copy
   std::vector<int> order;            // descriptor slot -> instance (grouped by size)
-  std::vector<int> cls_n, cls_first, cls_count;  // size classes over descriptor slots
+  std::vector<int> cls_n, cls_first, cls_count;  // launch classes (one grid size each) over descriptor slots
   std::vector<size_t> cls_prefix_off;            // offset of each class's EP_COUNT x (count+1) table
   std::vector<int> prefix;
   std::vector<void*> tw_by_size;     // twiddles per entry of kEnsSizes
@@ -424,28 +425,97 @@ int batch_bind(tg_batch* b, void* state, void* ws) {
   std::vector<EnsInst<R>> h(b->ninst);
   char* w = static_cast<char*>(ws);
   cpx<R>* st = static_cast<cpx<R>*>(state);
+  b->dense.assign(b->ninst, nullptr);
+  b->padded.assign(b->ninst, nullptr);
   for (int slot = 0; slot < b->ninst; ++slot) {
     const int i = b->order[slot];
-    const int N = b->n[i], M = N / 3, K = 2 * M + 1, KZ = M + 1;
-    const size_t vec = size_t(3) * K * K * KZ * sizeof(cpx<R>);
-    const size_t sA = size_t(6) * N * K * ops_table<R>(N)->kzp * sizeof(cpx<R>);
+    const int N = b->n[i];
+    const EnsTable<R>* t = ens_table<R>(N);
+    const size_t K = 2 * (N / 3) + 1;
+    const size_t vec = 3 * K * size_t(t->pa) * sizeof(cpx<R>);
+    const size_t sA = size_t(t->a_elems) * sizeof(cpx<R>);
     char* base = w + b->ws_off[i];
     EnsInst<R>& I = h[slot];
     I.n = N;
     I.nu = R(b->nu[i]);
     I.ks = R(b->ks);
-    I.u0 = st + b->state_off[i];
-    I.u1 = reinterpret_cast<cpx<R>*>(base);
-    I.acc = reinterpret_cast<cpx<R>*>(base + align256(vec));
-    I.A = reinterpret_cast<cpx<R>*>(base + 2 * align256(vec));
-    I.B = reinterpret_cast<cpx<R>*>(base + 2 * align256(vec) + align256(sA));
+    I.s0 = reinterpret_cast<cpx<R>*>(base);
+    I.u1 = reinterpret_cast<cpx<R>*>(base + align256(vec));
+    I.acc = reinterpret_cast<cpx<R>*>(base + 2 * align256(vec));
+    I.A = reinterpret_cast<cpx<R>*>(base + 3 * align256(vec));
+    I.B = reinterpret_cast<cpx<R>*>(base + 3 * align256(vec) + align256(sA));
     I.tw = static_cast<const cpx<R>*>(b->tw_by_size[ens_size_index(N)]);
+    b->dense[i] = st + b->state_off[i];
+    b->padded[i] = I.s0;
   }
   TG_HIP(hipMemcpy(b->d_inst, h.data(), h.size() * sizeof(EnsInst<R>), hipMemcpyHostToDevice));
   b->bound = true;
   return TG_OK;
 }
 
+// Copy every instance's caller-owned dense state into its plane-padded
+// working copy (to_padded) or back.
+int batch_copy_state(tg_batch* b, bool to_padded, hipStream_t s) {
+  for (int i = 0; i < b->ninst; ++i) {
+    const int N = b->n[i];
+    const size_t K = 2 * (N / 3) + 1, KZ = N / 3 + 1;
+    const size_t pa = b->prec == 64 ? size_t(ens_table<double>(N)->pa) : size_t(ens_table<float>(N)->pa);
+    const size_t row = K * KZ * b->cb, prow = pa * b->cb;
+    if (to_padded)
+      TG_HIP(hipMemcpy2DAsync(b->padded[i], prow, b->dense[i], row, row, 3 * K, hipMemcpyDeviceToDevice, s));
+    else
+      TG_HIP(hipMemcpy2DAsync(b->dense[i], row, b->padded[i], prow, row, 3 * K, hipMemcpyDeviceToDevice, s));
+  }
+  return TG_OK;
+}
+
+}  // namespace
+
+namespace {
+// The steps of tg_batch_rk4_steps on the instances' working copies.
+int batch_steps(tg_batch* b, int nsteps, double dt, int use_graph, hipStream_t s) {
+  auto step = [&](hipStream_t st, bool branches) {
+    return b->prec == 64 ? batch_launch_step<double>(b, dt, st, branches)
+                         : batch_launch_step<float>(b, dt, st, branches);
+  };
+  if (!use_graph) {
+    for (int i = 0; i < nsteps; ++i) TG_RET(step(s, false));
+    return TG_OK;
+  }
+  if (!b->exec || b->graph_dt != dt) {
+    if (b->exec) {
+      TG_HIP(hipGraphExecDestroy(b->exec));
+      b->exec = nullptr;
+    }
+    if (!b->cap) TG_HIP(hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking));
+    const size_t ncls = b->cls_n.size();
+    while (b->branch.size() + 1 < ncls) {
+      hipStream_t t = nullptr;
+      TG_HIP(hipStreamCreateWithFlags(&t, hipStreamNonBlocking));
+      b->branch.push_back(t);
+    }
+    while (b->fork.size() < ncls) {
+      hipEvent_t e = nullptr;
+      TG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      b->fork.push_back(e);
+    }
+    hipGraph_t g = nullptr;
+    TG_HIP(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal));
+    const int rc = step(b->cap, true);
+    hipError_t e = hipStreamEndCapture(b->cap, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return fail(TG_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiateWithFlags(&b->exec, g, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return fail(TG_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    b->graph_dt = dt;
+  }
+  for (int i = 0; i < nsteps; ++i) TG_HIP(hipGraphLaunch(b->exec, s));
+  return TG_OK;
+}
 }  // namespace
 
 extern "C" {
@@ -473,12 +543,19 @@ int tg_batch_create(int ninst, const int* n, const double* nu, int precision_bit
   b->ws_off.assign(ninst + 1, 0);
   for (int i = 0; i < ninst; ++i) {
     const size_t N = n[i], M = N / 3, K = 2 * M + 1, KZ = M + 1;
-    const size_t vec = 3 * K * K * KZ * b->cb;
     b->state_off[i + 1] = b->state_off[i] + (long long)(3 * K * K * KZ);
-    const size_t kzpad = precision_bits == 64 ? size_t(ops_table<double>(n[i])->kzp) : size_t(ops_table<float>(n[i])->kzp);
-    b->ws_off[i + 1] = b->ws_off[i] + 2 * align256(vec) + align256(6 * N * K * kzpad * b->cb) +
-                       align256(6 * N * N * kzpad * b->cb);
+    size_t pa, ae, be;
+    if (precision_bits == 64) {
+      const EnsTable<double>* t = ens_table<double>(n[i]);
+      pa = t->pa; ae = t->a_elems; be = t->b_elems;
+    } else {
+      const EnsTable<float>* t = ens_table<float>(n[i]);
+      pa = t->pa; ae = t->a_elems; be = t->b_elems;
+    }
+    const size_t vec = 3 * K * pa * b->cb;  // state, stage input, accumulator
+    b->ws_off[i + 1] = b->ws_off[i] + 3 * align256(vec) + align256(ae * b->cb) + align256(be * b->cb);
   }
+
   for (int k = 0; k < kNumEnsSizes; ++k) {
     int cnt = 0;
     for (int i = 0; i < ninst; ++i)
@@ -491,6 +568,25 @@ int tg_batch_create(int ninst, const int* n, const double* nu, int precision_bit
         ++cnt;
       }
     if (cnt) b->cls_count.push_back(cnt);
+  }
+  // A class holding more than 40% of the work (instances x N^3) is split in
+  // two halves, each its own launch sequence (and graph branch): while one
+  // half runs its instruction-bound z pass the other streams through a
+  // memory-bound pass.
+  {
+    double total = 0.0;
+    for (int i = 0; i < ninst; ++i) total += double(n[i]) * n[i] * n[i];
+    for (size_t c = 0; c < b->cls_n.size(); ++c) {
+      const double w = double(b->cls_count[c]) * b->cls_n[c] * b->cls_n[c] * b->cls_n[c];
+      if (b->cls_count[c] > 1 && w > 0.4 * total) {
+        const int h = b->cls_count[c] / 2;
+        b->cls_n.insert(b->cls_n.begin() + c + 1, b->cls_n[c]);
+        b->cls_first.insert(b->cls_first.begin() + c + 1, b->cls_first[c] + h);
+        b->cls_count.insert(b->cls_count.begin() + c + 1, b->cls_count[c] - h);
+        b->cls_count[c] = h;
+        ++c;
+      }
+    }
   }
   for (size_t c = 0; c < b->cls_n.size(); ++c) {
     const int cnt = b->cls_count[c];
@@ -567,52 +663,16 @@ int tg_batch_bind(tg_batch* b, void* state, void* ws) {
   return b->prec == 64 ? batch_bind<double>(b, state, ws) : batch_bind<float>(b, state, ws);
 }
 
+
 int tg_batch_rk4_steps(tg_batch* b, int nsteps, double dt, int use_graph, void* stream) {
   if (!b) return fail(TG_ERR_ARG, "batch is NULL");
   if (!b->bound) return fail(TG_ERR_ARG, "tg_batch_rk4_steps: call tg_batch_bind first");
   if (nsteps < 0) return fail(TG_ERR_ARG, "nsteps must be >= 0");
+  if (nsteps == 0) return TG_OK;
   hipStream_t s = static_cast<hipStream_t>(stream);
-  auto step = [&](hipStream_t st, bool branches) {
-    return b->prec == 64 ? batch_launch_step<double>(b, dt, st, branches)
-                         : batch_launch_step<float>(b, dt, st, branches);
-  };
-  if (!use_graph) {
-    for (int i = 0; i < nsteps; ++i) TG_RET(step(s, false));
-    return TG_OK;
-  }
-  if (!b->exec || b->graph_dt != dt) {
-    if (b->exec) {
-      TG_HIP(hipGraphExecDestroy(b->exec));
-      b->exec = nullptr;
-    }
-    if (!b->cap) TG_HIP(hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking));
-    const size_t ncls = b->cls_n.size();
-    while (b->branch.size() + 1 < ncls) {
-      hipStream_t t = nullptr;
-      TG_HIP(hipStreamCreateWithFlags(&t, hipStreamNonBlocking));
-      b->branch.push_back(t);
-    }
-    while (b->fork.size() < ncls) {
-      hipEvent_t e = nullptr;
-      TG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-      b->fork.push_back(e);
-    }
-    hipGraph_t g = nullptr;
-    TG_HIP(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal));
-    const int rc = step(b->cap, true);
-    hipError_t e = hipStreamEndCapture(b->cap, &g);
-    if (rc) {
-      if (g) (void)hipGraphDestroy(g);
-      return rc;
-    }
-    if (e != hipSuccess) return fail(TG_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-    e = hipGraphInstantiateWithFlags(&b->exec, g, 0);
-    (void)hipGraphDestroy(g);
-    if (e != hipSuccess) return fail(TG_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
-    b->graph_dt = dt;
-  }
-  for (int i = 0; i < nsteps; ++i) TG_HIP(hipGraphLaunch(b->exec, s));
-  return TG_OK;
+  TG_RET(batch_copy_state(b, true, s));
+  TG_RET(batch_steps(b, nsteps, dt, use_graph, s));
+  return batch_copy_state(b, false, s);
 }
 
 int tg_last_error(char* buf, size_t n) {

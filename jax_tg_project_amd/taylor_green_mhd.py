@@ -32,7 +32,14 @@ _TWO_PI = 2.0 * math.pi
 @dataclass(frozen=True)
 class TaylorGreenMhdConfig:
     """cuda/taylor_green_mhd/config.py:7-50 (+ lorentz_sign, initial_condition,
-    dtype selectors with reference-compatible defaults)."""
+    dtype selectors with reference-compatible defaults).
+
+    With lorentz_sign = -1 (the reference code's sign) the state
+    initial_condition="config" (b0 = -curl u_TG) is linearly unstable: at
+    n=128, nu = eta = 1/200, dt = 2e-3 a run from the float32 initial arrays
+    diverges within 300 steps in FP32 and in FP64, and the run loop raises
+    FloatingPointError.  lorentz_sign = +1 (the documented equation) keeps it
+    finite."""
     n: int = 128
     dt: float = 0.005
     reynolds: float = 1000.0

@@ -131,19 +131,23 @@ def test_run_loops_raise_on_non_finite_state(tg, mhd, monkeypatch):
         taylor_green_cuda.TaylorGreenSimulation(hcfg, make_plots=False).run()
 
 
-def test_mhd_fp32_reference_sign_diverges_at_the_128_config(mhd):
-    """b0 = -curl u_TG, nu = eta = 1/200, dt = 2e-3, N = 128 with the
-    reference code's Lorentz sign: the state is linearly unstable and FP32
-    round-off seeds a blow-up within 300 steps; the run loop reports it
-    instead of returning NaN.  FP64 stays finite over the same 300 steps."""
+def test_mhd_reference_sign_diverges_at_the_128_config(mhd):
+    """b0 = -curl u_TG, nu = eta = 1/200, dt = 2e-3, N = 128 from the
+    float32 initial arrays the reference builds: with the reference code's
+    Lorentz sign (-P(j x b), operators.py:88-92) the state is linearly
+    unstable and the run blows up within 300 steps in FP32 and in FP64; the
+    run loop reports it instead of returning NaN.  With the documented sign
+    (+ (curl B) x B, README_MHD.md:9) the same FP32 run stays finite."""
     re = LENGTH * 200.0
     base = dict(n=128, dt=2e-3, reynolds=re, magnetic_reynolds=re, total_time=0.6, save_every_time=0.05,
-                initial_condition="config", lorentz_sign=-1.0)
-    with pytest.raises(FloatingPointError, match="non-finite"):
-        mhd.TaylorGreenMhdSimulation(mhd.TaylorGreenMhdConfig(precision="fp32", **base), make_plots=False).run()
-    _, _, d = mhd.TaylorGreenMhdSimulation(mhd.TaylorGreenMhdConfig(precision="fp64", **base),
-                                          make_plots=False).run()
-    assert all(math.isfinite(v) for v in d.total_energies)
+                initial_condition="config")
+    for prec in ("fp32", "fp64"):
+        cfg = mhd.TaylorGreenMhdConfig(precision=prec, lorentz_sign=-1.0, **base)
+        with pytest.raises(FloatingPointError, match="non-finite"):
+            mhd.TaylorGreenMhdSimulation(cfg, make_plots=False).run()
+    cfg = mhd.TaylorGreenMhdConfig(precision="fp32", lorentz_sign=1.0, **base)
+    _, _, d = mhd.TaylorGreenMhdSimulation(cfg, make_plots=False).run()
+    assert len(d.times) == 13 and all(math.isfinite(v) for v in d.total_energies)
 
 
 # ---------------------------------------------------------------- MHD operator helpers
@@ -226,7 +230,7 @@ def test_mhd_simulation_run_matches_oracle_run(mhd, kind, sign):
     """TaylorGreenMhdSimulation.run (cuda/taylor_green_mhd/simulation.py:49-100)
     against the oracle's run loop from the same float32 initial arrays:
     record times exactly, E_k, E_m, Z relative 1e-12, H_c absolute 1e-13 of
-    the energy scale, divergence maxima at round-off."""
+    the energy scale, divergence maxima to 1e-12 absolute."""
     re = 120.0
     cfg = mhd.TaylorGreenMhdConfig(n=16, dt=0.004, reynolds=re, magnetic_reynolds=1.5 * re, total_time=0.1,
                                    save_every_time=0.02, initial_condition=kind, lorentz_sign=sign)
@@ -243,7 +247,9 @@ def test_mhd_simulation_run_matches_oracle_run(mhd, kind, sign):
         assert np.max(np.abs(got - want) / np.abs(want)) < 1e-12, key
     scale = max(ref["total_energies"])
     assert np.max(np.abs(np.array(d.cross_helicities) - np.array(ref["cross_helicities"]))) < 1e-13 * scale
-    assert max(d.divergence_u_max) < 1e-12 and max(d.divergence_b_max) < 1e-12
+    # the float32 initial arrays carry ~1e-7 of divergence; the maxima follow the oracle's
+    for key in ("divergence_u_max", "divergence_b_max"):
+        assert np.max(np.abs(np.array(getattr(d, key)) - np.array(ref[key]))) < 1e-12, key
     assert rel_l2(uh.cpu().numpy(), wu) < 1e-12 and rel_l2(bh.cpu().numpy(), wb) < 1e-12
 
 

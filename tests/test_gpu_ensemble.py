@@ -1,7 +1,7 @@
 """Batched ensemble: every instance of a mixed-size batch matches the FP64
 oracle (advective form where dealiasing is exact, rotational form for
-N = 48, 96), the batched kernels match the single-instance plan, and graph
-replays match eager execution on every replay."""
+N = 48, 96), the batched kernels match the single-instance plans to
+round-off, and graph replays match eager execution bitwise on every replay."""
 import numpy as np
 import pytest
 import torch
@@ -40,15 +40,19 @@ def test_ensemble_fp64_matches_oracle_per_instance():
         assert rel_l2(got, ref) < 1e-12, (N, rel_l2(got, ref))
 
 
-def test_ensemble_matches_single_instance_plan_bitwise():
-    ens, u0, nus = _ens(torch.float32)
+@pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-14), (torch.float32, 2e-6)], ids=["fp64", "fp32"])
+def test_ensemble_matches_single_instance_plans(dtype, tol):
+    """The batched dense-layout passes against the single-grid plans (same
+    FFT plans and formulas, different tiles and layouts: equal to round-off)."""
+    ens, u0, nus = _ens(dtype)
     ens.step(2, 5e-3, graph=False)
     for i, N in enumerate(SIZES):
         p = ens.plan(i)
         p.set_variants([0] * 5)
-        uh = p.forward(torch.tensor(u0[i], dtype=torch.float32, device="cuda"))
+        uh = p.forward(torch.tensor(u0[i], dtype=dtype, device="cuda"))
         p.rk4(uh, nsteps=2, dt=5e-3, nu=nus[i])
-        assert torch.equal(uh, ens.views[i]), N
+        err = (torch.linalg.vector_norm(uh - ens.views[i]) / torch.linalg.vector_norm(uh)).item()
+        assert err < tol, (N, err)
 
 
 def test_graph_replays_match_eager_every_replay():
