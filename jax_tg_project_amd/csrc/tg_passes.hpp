@@ -259,6 +259,17 @@ struct XInvBody {
             const cpx<R>* src = c < 3 ? su : sb;
             sv[c][j * RAD0 + m] = src[(c % 3) * cs + pt];
           }
+          if constexpr (MODE == XM_ADJQ) {
+            // q = alpha P(a) once per point (the three fields then only read it)
+            const R kx = ks * R(G::kint(ix)), kz = ks * R(iz);
+            const R k2 = kx * kx + ky * ky + kz * kz;
+            const R r = R(1) / ((k2 == R(0)) ? R(1) : k2);
+            const cpx<R> a0 = sv[0][j * RAD0 + m], a1 = sv[1][j * RAD0 + m], a2 = sv[2][j * RAD0 + m];
+            const cpx<R> kd = {(kx * a0.x + ky * a1.x + kz * a2.x) * r, (kx * a0.y + ky * a1.y + kz * a2.y) * r};
+            sv[0][j * RAD0 + m] = {alpha * (a0.x - kx * kd.x), alpha * (a0.y - kx * kd.y)};
+            sv[1][j * RAD0 + m] = {alpha * (a1.x - ky * kd.x), alpha * (a1.y - ky * kd.y)};
+            sv[2][j * RAD0 + m] = {alpha * (a2.x - kz * kd.x), alpha * (a2.y - kz * kd.y)};
+          }
         }
       }
     }
@@ -273,7 +284,9 @@ struct XInvBody {
         const int ix = ok ? G::compact_of(n) : 0;
         const R kx = ks * R(G::kint(ix)), kz = ks * R(iz);
         cpx<R> v;
-        if constexpr (PRE) {
+        if constexpr (PRE && MODE == XM_ADJQ) {
+          v = sv[f][slot];  // projected at preload
+        } else if constexpr (PRE) {
           v = xfield_reg<R, N, MODE, NS, E>(f, sv, slot, kx, ky, kz, alpha);
         } else {
           const size_t pt = (size_t(ix) * G::K + iy) * cpz + iz;
@@ -1246,12 +1259,13 @@ struct StageArgs {
 };
 
 // Adjoint stage bookkeeping of one compact point (the FM_ADJ contract of
-// StageArgs): v[0..2] is the forward-x output of the product's VJP through
-// u, v[3..5] the field whose curl adds to it.  Every load of the point is
-// issued before its stores; returns the point's term of the nu gradient.
+// StageArgs) given w = the forward-x output of the product's VJP through u
+// plus the curl of the field that adds to it (so mu = w - nu k^2 a_s).
+// Every load of the point is issued before its stores; returns the point's
+// term of the nu gradient.
 template <typename R>
-__device__ __forceinline__ double adj_point(const StageArgs<R>& a, size_t pt, size_t cs, R kx, R ky, R kz,
-                                            int iz, const cpx<R> (&v)[6]) {
+__device__ __forceinline__ double adj_point_w(const StageArgs<R>& a, size_t pt, size_t cs, R k2, int iz,
+                                              const cpx<R> (&w)[3]) {
   cpx<R> ar[3], lp[3], l0[3], xv[3];
   const bool uselp = a.first == 1 || !a.last;
 #pragma unroll
@@ -1262,19 +1276,14 @@ __device__ __forceinline__ double adj_point(const StageArgs<R>& a, size_t pt, si
   for (int c = 0; c < 3; ++c) l0[c] = a.first == 0 ? a.lacc[c * cs + pt] : cpx<R>{R(0), R(0)};
 #pragma unroll
   for (int c = 0; c < 3; ++c) xv[c] = a.xs ? a.xs[c * cs + pt] : cpx<R>{R(0), R(0)};
-  const R k2 = kx * kx + ky * ky + kz * kz;
   const R nk = a.nu * k2;
-  cpx<R> cb[3];
-  cb[0] = cmuli(cpx<R>{ky * v[5].x - kz * v[4].x, ky * v[5].y - kz * v[4].y});
-  cb[1] = cmuli(cpx<R>{kz * v[3].x - kx * v[5].x, kz * v[3].y - kx * v[5].y});
-  cb[2] = cmuli(cpx<R>{kx * v[4].x - ky * v[3].x, kx * v[4].y - ky * v[3].y});
   const double wk = (iz == 0 ? 1.0 : 2.0) * double(k2);
   double nus = 0.0;
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     const size_t o = c * cs + pt;
     const cpx<R> as = {a.alpha * ar[c].x, a.alpha * ar[c].y};
-    const cpx<R> mu = {v[c].x + cb[c].x - nk * as.x, v[c].y + cb[c].y - nk * as.y};
+    const cpx<R> mu = {w[c].x - nk * as.x, w[c].y - nk * as.y};
     cpx<R> acc = mu;
     if (a.first == 1) acc = {lp[c].x + mu.x, lp[c].y + mu.y};
     else if (a.first == 0) acc = {l0[c].x + mu.x, l0[c].y + mu.y};
@@ -1283,6 +1292,18 @@ __device__ __forceinline__ double adj_point(const StageArgs<R>& a, size_t pt, si
     if (!a.last) a.anext[o] = {a.clam * lp[c].x + a.cmu * mu.x, a.clam * lp[c].y + a.cmu * mu.y};
   }
   return nus;
+}
+
+// The same from the six forward-x outputs: v[0..2] the product's VJP
+// through u, v[3..5] the field whose curl adds to it.
+template <typename R>
+__device__ __forceinline__ double adj_point(const StageArgs<R>& a, size_t pt, size_t cs, R kx, R ky, R kz,
+                                            int iz, const cpx<R> (&v)[6]) {
+  cpx<R> w[3];
+  w[0] = cadd(v[0], cmuli(cpx<R>{ky * v[5].x - kz * v[4].x, ky * v[5].y - kz * v[4].y}));
+  w[1] = cadd(v[1], cmuli(cpx<R>{kz * v[3].x - kx * v[5].x, kz * v[3].y - kx * v[5].y}));
+  w[2] = cadd(v[2], cmuli(cpx<R>{kx * v[4].x - ky * v[3].x, kx * v[4].y - ky * v[3].y}));
+  return adj_point_w(a, pt, cs, kx * kx + ky * ky + kz * kz, iz, w);
 }
 
 // Leray projection n - k (k.n) / k2 (k2s: k2 with 0 -> 1).  FP64 divides
@@ -1511,6 +1532,41 @@ struct XRBody {
     const bool colok = izr < G::KZ;
     const int iz = colok ? izr : G::KZ - 1;
     stage_twiddles<R, N>(tws, tw);
+    if constexpr (KIND == XR_ADJ) {
+      // w = v + i k x p, folded as each p component leaves its transform: only
+      // the three w fields and one transformed field are live at a time
+      const R ky = a.ks * R(G::kint(iy)), kz = a.ks * R(iz);
+      cpx<R> w[3][E];
+#pragma unroll
+      for (int f = 0; f < 6; ++f) {
+        if (f) __syncthreads();  // the previous field's last LDS reads are done
+        auto ld = [&](int n, int, int) -> cpx<R> {
+          const cpx<R> v = gld<nt_mid<N>()>(&in[((size_t(f) * N + n) * G::K + iy) * kzp<R, N>() + iz]);
+          return colok ? v : cpx<R>{R(0), R(0)};
+        };
+        // (i k x p)_x = i (ky p_z - kz p_y), _y = i (kz p_x - kx p_z), _z = i (kx p_y - ky p_x)
+        auto st = [&](int n, int, cpx<R> p, int slot) {
+          if (f < 3) {
+            w[f][slot] = p;
+            return;
+          }
+          const R kx = a.ks * R(G::kint(G::kept(n) ? G::compact_of(n) : 0));
+          if (f == 3) {
+            w[1][slot] = cadd(w[1][slot], cmuli(cpx<R>{kz * p.x, kz * p.y}));
+            w[2][slot] = csub(w[2][slot], cmuli(cpx<R>{ky * p.x, ky * p.y}));
+          } else if (f == 4) {
+            w[0][slot] = csub(w[0][slot], cmuli(cpx<R>{kz * p.x, kz * p.y}));
+            w[2][slot] = cadd(w[2][slot], cmuli(cpx<R>{kx * p.x, kx * p.y}));
+          } else {
+            w[0][slot] = cadd(w[0][slot], cmuli(cpx<R>{ky * p.x, ky * p.y}));
+            w[1][slot] = csub(w[1][slot], cmuli(cpx<R>{kx * p.x, kx * p.y}));
+          }
+        };
+        fft_io<R, N, E, -1, 1, true>(buf, L, L, tws, ld, st);
+      }
+      adjoint_epilogue(buf, w, colok, t, iy, iz, a);
+      return;
+    }
     cpx<R> o[FF][E];
 #pragma unroll
     for (int f = 0; f < FF; ++f) {
@@ -1521,10 +1577,6 @@ struct XRBody {
       };
       auto st = [&](int, int, cpx<R> v, int slot) { o[f][slot] = v; };
       fft_io<R, N, E, -1, 1, true>(buf, L, L, tws, ld, st);
-    }
-    if constexpr (KIND == XR_ADJ) {
-      adjoint_epilogue(buf, o, colok, t, iy, iz, a);
-      return;
     }
     if (!colok) return;
     const int cpz = a.cpz ? a.cpz : G::KZ;
@@ -1577,11 +1629,11 @@ struct XRBody {
     }
   }
 
-  // Stage-VJP bookkeeping on the register-held lines (o[0..2]: the VJP of
-  // the product through u, o[3..5]: the field whose curl adds to it), as
-  // the FM_ADJ epilogue of XFBody states it; the block's share of the nu
+  // Stage-VJP bookkeeping on the register-held lines (w = the VJP of the
+  // product through u plus the curl of the field that adds to it), as the
+  // FM_ADJ epilogue of XFBody states it; the block's share of the nu
   // gradient goes to a.nupart[block].  Every thread reaches the barriers.
-  __device__ __forceinline__ static void adjoint_epilogue(cpx<R>* buf, const cpx<R> (&o)[FF][C::E], bool colok,
+  __device__ __forceinline__ static void adjoint_epilogue(cpx<R>* buf, const cpx<R> (&w)[3][C::E], bool colok,
                                                           int t, int iy, int iz, const StageArgs<R>& a) {
     constexpr int E = C::E;
     using Pl = typename PlanFor<N, E>::type;
@@ -1601,8 +1653,8 @@ struct XRBody {
           const size_t pt = (size_t(ix) * G::K + iy) * cpz + iz;
           const int sl = j * RL + m;
           const R kx = a.ks * R(G::kint(ix));
-          const cpx<R> v[6] = {o[0][sl], o[1][sl], o[2][sl], o[3][sl], o[4][sl], o[5][sl]};
-          nusum += adj_point(a, pt, cs, kx, ky, kz, iz, v);
+          const cpx<R> wv[3] = {w[0][sl], w[1][sl], w[2][sl]};
+          nusum += adj_point_w(a, pt, cs, kx * kx + ky * ky + kz * kz, iz, wv);
         }
       }
     }
@@ -1625,7 +1677,7 @@ struct XRBody {
 // compiler's allocation (capping them spills more than it gains).
 template <typename R, int KIND>
 constexpr int xr_waves_per_eu() {
-  return (sizeof(R) == 8 && KIND == XR_HYDRO) ? 3 : 1;
+  return (sizeof(R) == 8 && KIND == XR_HYDRO) ? 3 : (KIND == XR_ADJ ? 2 : 1);
 }
 
 template <typename R, int N, bool MHD>
