@@ -199,6 +199,33 @@ int tg_adjoint_forward(tg_plan* plan, const void* x0_hat, int nsteps, double dt,
 int tg_adjoint_backward(tg_plan* plan, int nsteps, double dt, double nu, int seg, void* grad_hat, double* dJdnu,
                         void* ws, void* adj_ws, void* stream);
 
+/* ---- State dump and restart (no counterpart in the reference, which keeps
+ * no state between runs; its default run, jax/taylor_green_jax.py:327-333,
+ * is 12.5M steps).  A checkpoint file holds a 128-byte little-endian header
+ *   "TGDNSCK1" | int32 version=1, n, precision_bits, mhd |
+ *   double domain_length, time, dt, nu, eta, lorentz_sign |
+ *   int64 step, compact_elems (K*K*KZ per component) | uint64 FNV-1a-64 of the payload | zero pad
+ * followed by the payload: u_hat (3 compact components, plan precision),
+ * then b_hat for MHD plans.  tg_checkpoint_write copies the device state to
+ * the host, writes "<path>.tmp" and renames it over <path>, so a crash never
+ * leaves a torn file; it synchronises `stream`. */
+#define TG_CKPT_HEADER_BYTES 128
+typedef struct tg_checkpoint_meta {
+  long long step;
+  double time, dt, nu, eta, lorentz_sign;
+} tg_checkpoint_meta;
+int tg_checkpoint_write(tg_plan* plan, const char* path, const void* u_hat, const void* b_hat,
+                        const tg_checkpoint_meta* meta, void* stream);
+/* Header of a checkpoint file (host only, no GPU needed): grid size,
+ * precision, equations, domain length and the metadata.  Fails with
+ * TG_ERR_ARG on a missing, short, foreign or corrupt (checksum) file. */
+int tg_checkpoint_info(const char* path, int* n, int* precision_bits, int* mhd, double* domain_length,
+                       tg_checkpoint_meta* meta);
+/* Load a checkpoint into u_hat (and b_hat) of a plan with the same n,
+ * precision, equations and domain length; synchronises `stream`. */
+int tg_checkpoint_read(tg_plan* plan, const char* path, void* u_hat, void* b_hat, tg_checkpoint_meta* meta,
+                       void* stream);
+
 /* Copy the calling thread's last error message into buf (NUL-terminated). */
 int tg_last_error(char* buf, size_t n);
 

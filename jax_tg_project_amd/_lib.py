@@ -22,6 +22,17 @@ TG_DIAG_COUNT = 11
 _lib = None
 
 _VP = ctypes.c_void_p
+
+
+class CheckpointMeta(ctypes.Structure):
+    """tg_checkpoint_meta of include/tgdns.h."""
+    _fields_ = [("step", ctypes.c_longlong), ("time", ctypes.c_double), ("dt", ctypes.c_double),
+                ("nu", ctypes.c_double), ("eta", ctypes.c_double), ("lorentz_sign", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 _SIGS = {
     "tg_version": ([], ctypes.c_int),
     "tg_supported_sizes": ([ctypes.POINTER(ctypes.c_int), ctypes.c_int], ctypes.c_int),
@@ -55,6 +66,11 @@ _SIGS = {
                             ctypes.POINTER(ctypes.c_double), _VP, _VP, _VP], ctypes.c_int),
     "tg_adjoint_backward": ([_VP, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, _VP,
                              ctypes.POINTER(ctypes.c_double), _VP, _VP, _VP], ctypes.c_int),
+    "tg_checkpoint_write": ([_VP, ctypes.c_char_p, _VP, _VP, ctypes.POINTER(CheckpointMeta), _VP], ctypes.c_int),
+    "tg_checkpoint_info": ([ctypes.c_char_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                            ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double),
+                            ctypes.POINTER(CheckpointMeta)], ctypes.c_int),
+    "tg_checkpoint_read": ([_VP, ctypes.c_char_p, _VP, _VP, ctypes.POINTER(CheckpointMeta), _VP], ctypes.c_int),
     "tg_last_error": ([ctypes.c_char_p, ctypes.c_size_t], ctypes.c_int),
 }
 PASS_NAMES = ("inv_x", "inv_y", "z", "fwd_y", "fwd_x")
@@ -98,6 +114,18 @@ def supported_sizes():
 
 def _check_rc(rc):
     _check(rc)
+
+
+def checkpoint_info(path):
+    """(n, precision_bits, mhd, domain_length, meta dict) of a checkpoint file
+    (host only: no GPU needed)."""
+    lib = load()
+    n, prec, mhd = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    L = ctypes.c_double()
+    meta = CheckpointMeta()
+    _check(lib.tg_checkpoint_info(os.fsencode(path), ctypes.byref(n), ctypes.byref(prec), ctypes.byref(mhd),
+                                  ctypes.byref(L), ctypes.byref(meta)))
+    return int(n.value), int(prec.value), bool(mhd.value), float(L.value), meta.as_dict()
 
 
 def _require_gpu():
@@ -265,6 +293,29 @@ class Plan:
                                                  _ptr(grad), ctypes.byref(d), _ptr(self.ws), _ptr(aws),
                                                  _stream()))
         return grad, float(d.value)
+
+    # -- state dump / restart ---------------------------------------------------
+    def checkpoint_write(self, path, u_hat, b_hat=None, step=0, time=0.0, dt=0.0, nu=0.0, eta=0.0,
+                         lorentz_sign=0.0):
+        self._spec(u_hat, "u_hat")
+        if self.mhd:
+            self._spec(b_hat, "b_hat")
+        meta = CheckpointMeta(int(step), float(time), float(dt), float(nu), float(eta), float(lorentz_sign))
+        with torch.cuda.device(self.device):
+            _check(self._lib.tg_checkpoint_write(self._h, os.fsencode(path), _ptr(u_hat), _ptr(b_hat),
+                                                 ctypes.byref(meta), _stream()))
+
+    def checkpoint_read(self, path, u_hat, b_hat=None):
+        """Load a checkpoint of this plan's grid, precision and equations into
+        u_hat (and b_hat); returns its metadata."""
+        self._spec(u_hat, "u_hat")
+        if self.mhd:
+            self._spec(b_hat, "b_hat")
+        meta = CheckpointMeta()
+        with torch.cuda.device(self.device):
+            _check(self._lib.tg_checkpoint_read(self._h, os.fsencode(path), _ptr(u_hat), _ptr(b_hat),
+                                                ctypes.byref(meta), _stream()))
+        return meta.as_dict()
 
     def variant_count(self):
         cnt = ctypes.c_int()

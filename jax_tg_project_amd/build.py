@@ -5,7 +5,8 @@
 The kernels of each (precision, grid size) are one translation unit
 (csrc/tg_inst.hip compiled with -DTG_INST_PREC / -DTG_INST_N); the C ABI is
 csrc/tgdns.hip.  The objects compile in parallel and link into one shared
-library.  Every call recompiles everything.
+library; csrc/tg_cli.cpp links against it into the native command-line
+solver taylor_green_hip.  Every call recompiles everything.
 """
 import concurrent.futures as cf
 import os
@@ -19,6 +20,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libtgdns.so")
+CLI = os.path.join(HERE, "taylor_green_hip")
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
@@ -88,6 +90,15 @@ def build(verbose=True, out=OUT):
             sys.stderr.write(" ".join(cmd) + "\n" + res.stdout + res.stderr)
             raise RuntimeError(f"link failed with code {res.returncode}")
         os.replace(tmp, out)
+        if out == OUT:
+            # the native command-line solver, linked against the library next to it
+            cli = [cc, "-O2", "-std=c++17", "-o", CLI + ".tmp", os.path.join(CSRC, "tg_cli.cpp"),
+                   "-L" + HERE, "-ltgdns", "-Wl,-rpath,$ORIGIN"]
+            cmd, res = _run(cli)
+            if res.returncode != 0:
+                sys.stderr.write(" ".join(cmd) + "\n" + res.stdout + res.stderr)
+                raise RuntimeError(f"building taylor_green_hip failed with code {res.returncode}")
+            os.replace(CLI + ".tmp", CLI)
     finally:
         shutil.rmtree(tmpdir, ignore_errors=True)
     return out

@@ -47,7 +47,8 @@ inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 struct tg_plan {
   int n, prec, mhd;
   int M, K, KZ;
-  double ks;          // 2 pi / L
+  double ks;          // 2 pi / L (exactly 1 for L = 2 pi)
+  double L;           // domain length
   void* tw;           // N complex twiddles exp(-2 pi i j / N) in plan precision
   size_t cb;          // bytes per complex element
   size_t compact;     // complex elements per compact component

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <algorithm>
@@ -126,6 +127,7 @@ int tg_plan_create(int n, int precision_bits, int mhd, double domain_length, tg_
   p->KZ = p->M + 1;
   p->ks = 2.0 * M_PI / domain_length;
   if (domain_length == 2.0 * M_PI) p->ks = 1.0;
+  p->L = domain_length;
   p->cb = precision_bits == 64 ? 16 : 8;
   p->compact = size_t(p->K) * p->K * p->KZ;
   const size_t N = n;
@@ -673,6 +675,127 @@ int tg_batch_rk4_steps(tg_batch* b, int nsteps, double dt, int use_graph, void* 
   TG_RET(batch_copy_state(b, true, s));
   TG_RET(batch_steps(b, nsteps, dt, use_graph, s));
   return batch_copy_state(b, false, s);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- checkpoints
+namespace {
+
+struct CkptHeader {
+  char magic[8];
+  int32_t version, n, prec, mhd;
+  double L, time, dt, nu, eta, lorentz;
+  int64_t step, compact;
+  uint64_t checksum;
+  char pad[TG_CKPT_HEADER_BYTES - 8 - 4 * 4 - 6 * 8 - 2 * 8 - 8];
+};
+static_assert(sizeof(CkptHeader) == TG_CKPT_HEADER_BYTES, "checkpoint header is 128 bytes");
+constexpr char kCkptMagic[8] = {'T', 'G', 'D', 'N', 'S', 'C', 'K', '1'};
+
+uint64_t fnv1a(const unsigned char* p, size_t n, uint64_t h = 1469598103934665603ull) {
+  for (size_t i = 0; i < n; ++i) {
+    h ^= p[i];
+    h *= 1099511628211ull;
+  }
+  return h;
+}
+
+int read_header(const char* path, CkptHeader* h, std::vector<unsigned char>* payload) {
+  if (!path) return fail(TG_ERR_ARG, "checkpoint path is NULL");
+  FILE* f = fopen(path, "rb");
+  if (!f) return fail(TG_ERR_ARG, std::string("cannot open checkpoint ") + path);
+  const size_t got = fread(h, 1, sizeof(*h), f);
+  if (got != sizeof(*h) || memcmp(h->magic, kCkptMagic, 8) != 0 || h->version != 1) {
+    fclose(f);
+    return fail(TG_ERR_ARG, std::string(path) + " is not a libtgdns checkpoint (bad header)");
+  }
+  const size_t cb = h->prec == 64 ? 16 : 8;
+  const size_t bytes = size_t(h->compact) * 3 * cb * (h->mhd ? 2 : 1);
+  std::vector<unsigned char> buf(bytes);
+  const size_t pn = bytes ? fread(buf.data(), 1, bytes, f) : 0;
+  const bool extra = fgetc(f) != EOF;
+  fclose(f);
+  if (pn != bytes || extra) return fail(TG_ERR_ARG, std::string(path) + ": payload size does not match the header");
+  if (fnv1a(buf.data(), bytes) != h->checksum) return fail(TG_ERR_ARG, std::string(path) + ": checksum mismatch");
+  if (payload) payload->swap(buf);
+  return TG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tg_checkpoint_write(tg_plan* p, const char* path, const void* u_hat, const void* b_hat,
+                        const tg_checkpoint_meta* meta, void* stream) {
+  TG_RET(check_plan(p));
+  if (!path || !u_hat || !meta) return fail(TG_ERR_ARG, "tg_checkpoint_write: NULL argument");
+  if (p->mhd && !b_hat) return fail(TG_ERR_ARG, "tg_checkpoint_write: MHD plan needs b_hat");
+  const size_t vec = 3 * p->compact * p->cb;
+  std::vector<unsigned char> buf(vec * (p->mhd ? 2 : 1));
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  TG_HIP(hipMemcpyAsync(buf.data(), u_hat, vec, hipMemcpyDeviceToHost, s));
+  if (p->mhd) TG_HIP(hipMemcpyAsync(buf.data() + vec, b_hat, vec, hipMemcpyDeviceToHost, s));
+  TG_HIP(hipStreamSynchronize(s));
+  CkptHeader h{};
+  memcpy(h.magic, kCkptMagic, 8);
+  h.version = 1;
+  h.n = p->n;
+  h.prec = p->prec;
+  h.mhd = p->mhd;
+  h.L = p->L;
+  h.time = meta->time;
+  h.dt = meta->dt;
+  h.nu = meta->nu;
+  h.eta = meta->eta;
+  h.lorentz = meta->lorentz_sign;
+  h.step = meta->step;
+  h.compact = (int64_t)p->compact;
+  h.checksum = fnv1a(buf.data(), buf.size());
+  const std::string tmp = std::string(path) + ".tmp";
+  FILE* f = fopen(tmp.c_str(), "wb");
+  if (!f) return fail(TG_ERR_ARG, "cannot create " + tmp);
+  const bool ok = fwrite(&h, 1, sizeof(h), f) == sizeof(h) && fwrite(buf.data(), 1, buf.size(), f) == buf.size();
+  const bool closed = fclose(f) == 0;
+  if (!ok || !closed || rename(tmp.c_str(), path) != 0) {
+    remove(tmp.c_str());
+    return fail(TG_ERR_ARG, std::string("writing checkpoint ") + path + " failed");
+  }
+  return TG_OK;
+}
+
+int tg_checkpoint_info(const char* path, int* n, int* precision_bits, int* mhd, double* domain_length,
+                       tg_checkpoint_meta* meta) {
+  CkptHeader h;
+  TG_RET(read_header(path, &h, nullptr));
+  if (n) *n = h.n;
+  if (precision_bits) *precision_bits = h.prec;
+  if (mhd) *mhd = h.mhd;
+  if (domain_length) *domain_length = h.L;
+  if (meta) *meta = tg_checkpoint_meta{h.step, h.time, h.dt, h.nu, h.eta, h.lorentz};
+  return TG_OK;
+}
+
+int tg_checkpoint_read(tg_plan* p, const char* path, void* u_hat, void* b_hat, tg_checkpoint_meta* meta,
+                       void* stream) {
+  TG_RET(check_plan(p));
+  if (!u_hat) return fail(TG_ERR_ARG, "tg_checkpoint_read: NULL u_hat");
+  if (p->mhd && !b_hat) return fail(TG_ERR_ARG, "tg_checkpoint_read: MHD plan needs b_hat");
+  CkptHeader h;
+  std::vector<unsigned char> buf;
+  TG_RET(read_header(path, &h, &buf));
+  const double L = p->L;
+  if (h.n != p->n || h.prec != p->prec || h.mhd != p->mhd || std::fabs(h.L - L) > 1e-12 * L)
+    return fail(TG_ERR_ARG, std::string(path) + ": checkpoint of N=" + std::to_string(h.n) + ", " +
+                                std::to_string(h.prec) + "-bit, mhd=" + std::to_string(h.mhd) +
+                                " does not match the plan");
+  const size_t vec = 3 * p->compact * p->cb;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  TG_HIP(hipMemcpyAsync(u_hat, buf.data(), vec, hipMemcpyHostToDevice, s));
+  if (p->mhd) TG_HIP(hipMemcpyAsync(b_hat, buf.data() + vec, vec, hipMemcpyHostToDevice, s));
+  TG_HIP(hipStreamSynchronize(s));
+  if (meta) *meta = tg_checkpoint_meta{h.step, h.time, h.dt, h.nu, h.eta, h.lorentz};
+  return TG_OK;
 }
 
 int tg_last_error(char* buf, size_t n) {

@@ -60,6 +60,20 @@ class TaylorGreenSolver:
     def diagnostics(self, divmax=True):
         return self.plan.diagnostics(self.u_hat, divmax=divmax)
 
+    def save(self, path, step=0, time=0.0, dt=0.0):
+        """Write the compact state and (step, time, dt, nu) to a checkpoint file."""
+        self.plan.checkpoint_write(path, self.u_hat, step=step, time=time, dt=dt, nu=self.nu)
+
+    @classmethod
+    def load(cls, path, device=None):
+        """A solver restored from a checkpoint file, and the file's metadata."""
+        from ._lib import checkpoint_info
+        n, prec, mhd, L, meta = checkpoint_info(path)
+        if mhd:
+            raise ValueError(f"{path} holds an MHD state: use MhdSolver.load")
+        s = cls(n, meta["nu"], dtype=torch.float64 if prec == 64 else torch.float32, L=L, device=device)
+        return s, s.plan.checkpoint_read(path, s.u_hat)
+
 
 class MhdSolver:
     """MHD state (u_hat, b_hat), each (3, K, K, KZ), advanced by classic RK4.
@@ -97,3 +111,19 @@ class MhdSolver:
 
     def diagnostics(self, divmax=True):
         return self.plan.diagnostics(self.u_hat, self.b_hat, divmax=divmax)
+
+    def save(self, path, step=0, time=0.0, dt=0.0):
+        """Write (u_hat, b_hat) and (step, time, dt, nu, eta, lorentz_sign) to a checkpoint file."""
+        self.plan.checkpoint_write(path, self.u_hat, self.b_hat, step=step, time=time, dt=dt, nu=self.nu,
+                                   eta=self.eta, lorentz_sign=self.lorentz_sign)
+
+    @classmethod
+    def load(cls, path, device=None):
+        """A solver restored from a checkpoint file, and the file's metadata."""
+        from ._lib import checkpoint_info
+        n, prec, mhd, L, meta = checkpoint_info(path)
+        if not mhd:
+            raise ValueError(f"{path} holds a hydro state: use TaylorGreenSolver.load")
+        s = cls(n, meta["nu"], meta["eta"], dtype=torch.float64 if prec == 64 else torch.float32, L=L,
+                device=device, lorentz_sign=meta["lorentz_sign"])
+        return s, s.plan.checkpoint_read(path, s.u_hat, s.b_hat)

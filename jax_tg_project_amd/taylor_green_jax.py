@@ -351,3 +351,30 @@ def run_simulation(N=16, dt=0.005, nu=None, reynolds=None, total_time=0.5, save_
     if return_diagnostics:
         return u, energies, diagnostics
     return u, energies
+
+
+def main(argv=None):
+    """jax/taylor_green_jax.py:320-333 (python -m jax_tg_project_amd.taylor_green_jax): report the
+    device and run the reference's default case (N=256, dt=0.005, Re=1000,
+    total time 10 * 2 pi * 1000, saves every 0.05), each overridable."""
+    import argparse
+    parser = argparse.ArgumentParser(description="3D decaying Taylor-Green vortex on the MI355X HIP kernels")
+    parser.add_argument("--n", type=int, default=256)
+    parser.add_argument("--dt", type=float, default=0.005)
+    parser.add_argument("--reynolds", type=float, default=1000.0)
+    parser.add_argument("--total-time", type=float, default=10 * 2 * math.pi * 1000)
+    parser.add_argument("--save-every-time", type=float, default=0.05)
+    parser.add_argument("--precision", choices=("fp64", "fp32"), default="fp64")
+    parser.add_argument("--no-plots", action="store_true")
+    args = parser.parse_args(argv)
+    if torch.cuda.is_available():
+        print("GPU devices:", [torch.cuda.get_device_name(i) for i in range(torch.cuda.device_count())])
+    else:
+        print("WARNING: No GPU detected by torch. Check the ROCm setup before scaling up.")
+    return run_simulation(N=args.n, dt=args.dt, reynolds=args.reynolds, total_time=args.total_time,
+                          save_every_time=args.save_every_time, make_plots=not args.no_plots,
+                          dtype=torch.float64 if args.precision == "fp64" else torch.float32)
+
+
+if __name__ == "__main__":
+    main()

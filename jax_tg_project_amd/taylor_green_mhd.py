@@ -508,3 +508,32 @@ def run_simulation(n=128, dt=0.005, reynolds=1000.0, magnetic_reynolds=1000.0, m
     _, _, diagnostics = sim.run()
     u, b = sim.solver.fields()
     return u, b, diagnostics
+
+
+def main(argv=None):
+    """cuda/taylor_green_mhd/cli.py:7-29 (python -m jax_tg_project_amd.taylor_green_mhd), plus
+    --lorentz-sign, --initial-condition and --precision selectors."""
+    import argparse
+    parser = argparse.ArgumentParser(description="3D Taylor-Green MHD vortex on the MI355X HIP kernels")
+    parser.add_argument("--n", type=int, default=128)
+    parser.add_argument("--dt", type=float, default=0.005)
+    parser.add_argument("--reynolds", type=float, default=1000.0)
+    parser.add_argument("--magnetic-reynolds", type=float, default=1000.0)
+    parser.add_argument("--magnetic-amplitude", type=float, default=0.5)
+    parser.add_argument("--total-time", type=float, default=1.0)
+    parser.add_argument("--save-every-time", type=float, default=0.05)
+    parser.add_argument("--no-plots", action="store_true")
+    parser.add_argument("--lorentz-sign", type=float, default=-1.0)
+    parser.add_argument("--initial-condition", choices=("reference", "config"), default="reference")
+    parser.add_argument("--precision", choices=("fp64", "fp32"), default="fp64")
+    args = parser.parse_args(argv)
+    config = TaylorGreenMhdConfig(n=args.n, dt=args.dt, reynolds=args.reynolds,
+                                  magnetic_reynolds=args.magnetic_reynolds,
+                                  magnetic_amplitude=args.magnetic_amplitude, total_time=args.total_time,
+                                  save_every_time=args.save_every_time, lorentz_sign=args.lorentz_sign,
+                                  initial_condition=args.initial_condition, precision=args.precision)
+    return TaylorGreenMhdSimulation(config, make_plots=not args.no_plots).run()
+
+
+if __name__ == "__main__":
+    main()

@@ -169,3 +169,60 @@ def test_bench_refuses_multi_gpu():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+def _fnv1a(data):
+    h = 1469598103934665603
+    for b in data:
+        h ^= b
+        h = (h * 1099511628211) & 0xFFFFFFFFFFFFFFFF
+    return h
+
+
+def _write_ckpt(path, payload, n=8, prec=64, mhd=0, L=2 * np.pi, step=7, time=0.07, dt=0.01, nu=0.02, magic=b"TGDNSCK1",
+                checksum=None, compact=None):
+    import struct
+    M = n // 3
+    compact = (2 * M + 1) ** 2 * (M + 1) if compact is None else compact
+    head = struct.pack("<8s4i6d2qQ", magic, 1, n, prec, mhd, L, time, dt, nu, 0.0, 0.0, step, compact,
+                       _fnv1a(payload) if checksum is None else checksum)
+    with open(path, "wb") as f:
+        f.write(head + b"\0" * (128 - len(head)) + payload)
+
+
+def test_checkpoint_header_parsing_and_corruption(tmp_path):
+    """tg_checkpoint_info is host code: the 128-byte header, the FNV-1a-64
+    payload checksum, and the rejection of foreign, truncated and corrupted
+    files (include/tgdns.h documents the format)."""
+    from jax_tg_project_amd import _lib
+    rng = np.random.default_rng(0)
+    payload = rng.bytes(3 * 75 * 16)  # N=8: K=5, KZ=3 -> 75 complex per component, FP64
+    good = tmp_path / "good.tgck"
+    _write_ckpt(good, payload)
+    n, prec, mhd, L, meta = _lib.checkpoint_info(str(good))
+    assert (n, prec, mhd) == (8, 64, False) and abs(L - 2 * np.pi) < 1e-15
+    assert meta["step"] == 7 and meta["time"] == 0.07 and meta["dt"] == 0.01 and meta["nu"] == 0.02
+    bad = bytearray(payload)
+    bad[100] ^= 1
+    _write_ckpt(tmp_path / "flip.tgck", bytes(bad), checksum=_fnv1a(payload))
+    with pytest.raises(_lib.TgError, match="checksum"):
+        _lib.checkpoint_info(str(tmp_path / "flip.tgck"))
+    _write_ckpt(tmp_path / "short.tgck", payload[:-16], checksum=_fnv1a(payload))
+    with pytest.raises(_lib.TgError, match="payload size"):
+        _lib.checkpoint_info(str(tmp_path / "short.tgck"))
+    _write_ckpt(tmp_path / "magic.tgck", payload, magic=b"NOTACKPT")
+    with pytest.raises(_lib.TgError, match="not a libtgdns checkpoint"):
+        _lib.checkpoint_info(str(tmp_path / "magic.tgck"))
+    with pytest.raises(_lib.TgError, match="cannot open"):
+        _lib.checkpoint_info(str(tmp_path / "missing.tgck"))
+
+
+def test_native_cli_argument_errors():
+    """taylor_green_hip validates its arguments before touching the GPU
+    (cuda/taylor_green_cuda.cu:125: usage and exit status 1)."""
+    import subprocess
+    exe = os.path.join(ROOT, "jax_tg_project_amd", "taylor_green_hip")
+    for args in (["3"], ["16", "-0.1"], ["16", "0.01", "100", "1", "0.05", "extra"], ["--bogus"],
+                 ["16", "0.01", "100", "1", "0.05", "--precision", "16"]):
+        r = subprocess.run([exe] + args, capture_output=True, text=True)
+        assert r.returncode == 1 and "Usage:" in r.stderr, (args, r.returncode, r.stderr)
