@@ -128,6 +128,20 @@ int tg_rk4_steps(tg_plan* plan, void* u_hat, void* b_hat, int nsteps, double dt,
 int tg_diagnostics(tg_plan* plan, const void* u_hat, const void* b_hat, int with_divmax,
                    double* out, void* ws, void* stream);
 
+/* tg_diagnostics for callers that cannot allocate (no counterpart in the
+ * reference): code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc.  Same sums and out[] layout as tg_diagnostics, but
+ * no workspace argument: every intermediate lives in a 24 MiB device array
+ * inside the library (one per GPU, kept until the process ends), and the
+ * TG_DIAG_COUNT results are written to the DEVICE array out_dev on
+ * `stream` without a host copy or synchronisation.  Nothing needs clearing
+ * between calls; calls that overlap on one GPU (other streams, other host
+ * threads) share that array and race, so serialise them.  with_divmax needs
+ * two single-field intermediates: N <= 128 in FP64 and FP32; larger grids
+ * fail with TG_ERR_UNSUPPORTED (without with_divmax any grid works). */
+int tg_diagnostics_noalloc(tg_plan* plan, const void* u_hat, const void* b_hat, int with_divmax, double* out_dev,
+                           void* stream);
+
 /* Per-pass timing hook (no counterpart in the reference: measurement only).
  * While enabled, every pass launch of tg_rhs / tg_rk4_steps is bracketed by
  * two HIP events recorded on the launch stream; tg_timing_read waits for the

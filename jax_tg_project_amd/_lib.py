@@ -48,6 +48,7 @@ _SIGS = {
                       ctypes.c_double, ctypes.c_double, _VP, _VP], ctypes.c_int),
     "tg_diagnostics": ([_VP, _VP, _VP, ctypes.c_int, ctypes.POINTER(ctypes.c_double), _VP, _VP],
                        ctypes.c_int),
+    "tg_diagnostics_noalloc": ([_VP, _VP, _VP, ctypes.c_int, _VP, _VP], ctypes.c_int),
     "tg_plan_variants": ([_VP, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                           ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
     "tg_timing_enable": ([_VP, ctypes.c_int], ctypes.c_int),
@@ -316,6 +317,23 @@ class Plan:
             _check(self._lib.tg_checkpoint_read(self._h, os.fsencode(path), _ptr(u_hat), _ptr(b_hat),
                                                 ctypes.byref(meta), _stream()))
         return meta.as_dict()
+
+    def diagnostics_noalloc(self, u_hat, b_hat=None, divmax=True, out=None):
+        """tg_diagnostics_noalloc: the TG_DIAG_COUNT values into the float64 device
+        tensor `out` (allocated here when None), on the current stream, without
+        synchronising and without library allocations."""
+        self._spec(u_hat, "u_hat")
+        if b_hat is not None:
+            self._spec(b_hat, "b_hat")
+        if out is None:
+            out = torch.empty(TG_DIAG_COUNT, dtype=torch.float64, device=self.device)
+        if out.dtype != torch.float64 or out.numel() < TG_DIAG_COUNT or not out.is_contiguous() \
+                or out.device != self.device:
+            raise ValueError(f"out: expected a contiguous float64 tensor of {TG_DIAG_COUNT} on {self.device}")
+        with torch.cuda.device(self.device):
+            _check(self._lib.tg_diagnostics_noalloc(self._h, _ptr(u_hat), _ptr(b_hat), int(bool(divmax)), _ptr(out),
+                                                    _stream()))
+        return out
 
     def variant_count(self):
         cnt = ctypes.c_int()

@@ -589,6 +589,52 @@ struct Ops {
     *dJdnu = -(sum + comp) / n3;
     return TG_OK;
   }
+  // Scratch bytes diagnostics_dev needs (partials, sums, maxima and, with
+  // divmax, one field of each inverse-transform intermediate).
+  static size_t diag_scratch_bytes(const tg_plan* p, int divmax) {
+    size_t b = align256(size_t(p->nDiagBlk) * NDIAG * sizeof(double)) + align256((NDIAG + 2) * sizeof(double)) +
+               align256(2 * size_t(p->nMax) * sizeof(R));
+    if (divmax)
+      b += align256(size_t(N) * p->K * kzp<R, N>() * sizeof(cpx<R>)) +
+           align256(size_t(N) * N * kzp<R, N>() * sizeof(cpx<R>));
+    return b;
+  }
+  // tg_diagnostics with every intermediate in `scratch` and the normalised
+  // results written to the device array out_dev: no allocation, no host copy,
+  // no synchronisation.
+  static int diagnostics_dev(const tg_plan* p, const cpx<R>* u, const cpx<R>* b, int divmax, double* out_dev,
+                             void* scratch, hipStream_t s) {
+    using Lc = Launch<R, N>;
+    const cpx<R>* tw = static_cast<const cpx<R>*>(p->tw);
+    char* c = static_cast<char*>(scratch);
+    double* part = reinterpret_cast<double*>(c);
+    c += align256(size_t(p->nDiagBlk) * NDIAG * sizeof(double));
+    double* sums = reinterpret_cast<double*>(c);
+    c += align256((NDIAG + 2) * sizeof(double));
+    R* maxu = reinterpret_cast<R*>(c);
+    R* maxb = maxu + p->nMax;
+    c += align256(2 * size_t(p->nMax) * sizeof(R));
+    cpx<R>* A = reinterpret_cast<cpx<R>*>(c);
+    c += align256(size_t(N) * p->K * kzp<R, N>() * sizeof(cpx<R>));
+    cpx<R>* B = reinterpret_cast<cpx<R>*>(c);
+    TG_RET(Lc::diag(u, b, part, p->nDiagBlk, p->ks, s));
+    if (divmax) {
+      TG_RET(Lc::inv_x(XM_DIV, 0, u, nullptr, A, tw, R(p->ks), s));
+      TG_RET(Lc::y(+1, 0, 1, A, B, tw, s));
+      TG_RET(Lc::z(ZM_MAXABS, 0, B, nullptr, nullptr, maxu, tw, R(0), s));
+      if (b) {
+        TG_RET(Lc::inv_x(XM_DIV, 0, b, nullptr, A, tw, R(p->ks), s));
+        TG_RET(Lc::y(+1, 0, 1, A, B, tw, s));
+        TG_RET(Lc::z(ZM_MAXABS, 0, B, nullptr, nullptr, maxb, tw, R(0), s));
+      }
+    }
+    hipLaunchKernelGGL((k_diag_final<R>), dim3(1), dim3(64), 0, s, part, p->nDiagBlk, divmax ? maxu : nullptr,
+                       (divmax && b) ? maxb : nullptr, divmax ? p->nMax : 0, sums);
+    TG_LAUNCHED();
+    hipLaunchKernelGGL(k_diag_normalize<0>, dim3(1), dim3(64), 0, s, sums, p->n, out_dev);
+    TG_LAUNCHED();
+    return TG_OK;
+  }
   static int diagnostics(const tg_plan* p, const cpx<R>* u, const cpx<R>* b, int divmax, double* out,
                          void* ws, hipStream_t s) {
     using Lc = Launch<R, N>;
@@ -646,6 +692,8 @@ struct OpsTable {
                      hipStream_t);
   int (*adj_backward)(const tg_plan*, int, double, double, int, cpx<R>*, double*, void*, void*, hipStream_t);
   int (*diagnostics)(const tg_plan*, const cpx<R>*, const cpx<R>*, int, double*, void*, hipStream_t);
+  size_t (*diag_scratch_bytes)(const tg_plan*, int);
+  int (*diagnostics_dev)(const tg_plan*, const cpx<R>*, const cpx<R>*, int, double*, void*, hipStream_t);
 };
 
 template <typename R, int N>
@@ -663,6 +711,8 @@ OpsTable<R> make_table() {
   t.adj_forward = &O::adj_forward;
   t.adj_backward = &O::adj_backward;
   t.diagnostics = &O::diagnostics;
+  t.diag_scratch_bytes = &O::diag_scratch_bytes;
+  t.diagnostics_dev = &O::diagnostics_dev;
   return t;
 }
 

@@ -1851,4 +1851,20 @@ __global__ __launch_bounds__(64) void k_diag_final(const double* __restrict__ pa
   }
 }
 
+// Normalised diagnostics on the device (the tg_diagnostics out[] layout):
+// sums / N^6 (with 1/2 for the energies and the enstrophy), the divergence
+// maxima and the non-finite count.
+template <int UNUSED = 0>
+__global__ __launch_bounds__(64) void k_diag_normalize(const double* __restrict__ sums, int n,
+                                                       double* __restrict__ out) {
+  const int q = threadIdx.x;
+  const double n3 = double(n) * n * n;
+  const double n6 = n3 * n3;
+  if (q == 0 || q == 1 || q == 3) out[q] = 0.5 * sums[q] / n6;
+  else if (q == 2 || (q >= 4 && q <= 7)) out[q] = sums[q] / n6;
+  else if (q == 8) out[8] = sums[NDIAG];
+  else if (q == 9) out[9] = sums[NDIAG + 1];
+  else if (q == 10) out[10] = sums[8];
+}
+
 }  // namespace tg

@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstring>
 #include <algorithm>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -675,6 +676,60 @@ int tg_batch_rk4_steps(tg_batch* b, int nsteps, double dt, int use_graph, void* 
   TG_RET(batch_copy_state(b, true, s));
   TG_RET(batch_steps(b, nsteps, dt, use_graph, s));
   return batch_copy_state(b, false, s);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- allocation-free diagnostics
+// tg_diagnostics_noalloc keeps every intermediate in this file-scope device
+// array (one copy per GPU, held until the process ends).  Host code reaches
+// it only through the per-device address from hipGetSymbolAddress.
+namespace {
+
+constexpr size_t kScratchBytes = size_t(24) << 20;
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+void* g_scratch_ptr[kMaxDevices] = {};
+std::mutex g_scratch_mutex;
+
+// Address of `scratch` on the current device (looked up once per device).
+int scratch_ptr(void** out) {
+  int dev = 0;
+  TG_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(TG_ERR_UNSUPPORTED, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    TG_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return fail(TG_ERR_HIP, "hipGetSymbolAddress(scratch) returned NULL");
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return TG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tg_diagnostics_noalloc(tg_plan* p, const void* u_hat, const void* b_hat, int with_divmax, double* out_dev,
+                           void* stream) {
+  TG_RET(check_plan(p));
+  if (!u_hat || !out_dev) return fail(TG_ERR_ARG, "tg_diagnostics_noalloc: NULL buffer");
+  return with_table(p, [&](const auto& T, auto* rt) {
+    using R = std::remove_pointer_t<decltype(rt)>;
+    const size_t need = T.diag_scratch_bytes(p, with_divmax);
+    if (need > kScratchBytes)
+      return fail(TG_ERR_UNSUPPORTED, "tg_diagnostics_noalloc: N=" + std::to_string(p->n) + " needs " +
+                                          std::to_string(need) + " bytes of scratch" +
+                                          (with_divmax ? " with with_divmax" : "") + ", more than the " +
+                                          std::to_string(kScratchBytes) + " it holds; use tg_diagnostics");
+    void* sp = nullptr;
+    TG_RET(scratch_ptr(&sp));
+    return T.diagnostics_dev(p, static_cast<const cpx<R>*>(u_hat), static_cast<const cpx<R>*>(b_hat), with_divmax,
+                             out_dev, sp, static_cast<hipStream_t>(stream));
+  });
 }
 
 }  // extern "C"
