@@ -182,7 +182,7 @@ struct Launch {
   static int z_adj(const cpx<R>* in, const cpx<R>* qin, cpx<R>* out, const cpx<R>* tw, hipStream_t s) {
     if constexpr (adj_reg_z<N>()) {
       constexpr int WPB = ZwDefault<N>::WPB;
-      using C = CfgZW<R, N, 8, WPB>;
+      using C = CfgZW<R, N, adj_ze<R>(), WPB>;
       hipLaunchKernelGGL((k_zw_adj<R, N, WPB>), dim3(N * N / C::LINES), dim3(C::NT), 0, s, in, qin, out, tw);
       TG_LAUNCHED();
       return TG_OK;
@@ -455,20 +455,23 @@ struct Ops {
     return L;
   }
   // `len` RK4 steps from cur keeping the four stage states of step j in
-  // st[4j .. 4j+3] (stage 1 = the step's input state).
+  // st[4j .. 4j+3] (stage 1 = the step's input state); cur ends as the
+  // state after the last step.
   static int steps_keeping_stages(const tg_plan* p, cpx<R>* cur, cpx<R>* st, size_t vstride, int len, double dt,
                                   double nu, void* ws, hipStream_t s, cpx<R>* slots = nullptr,
                                   size_t sstride = 0) {
     Ptrs<R> w = carve<R>(p, ws);
     const size_t bytes = size_t(3) * p->K * p->K * kzp<R, N>() * sizeof(cpx<R>);
+    if (len > 0) TG_HIP(hipMemcpyAsync(st, cur, bytes, hipMemcpyDeviceToDevice, s));
     for (int j = 0; j < len; ++j) {
+      // step j starts from st[4j]; its last stage writes the next step's
+      // input slot directly (cur only after the last step)
       cpx<R>* S = st + size_t(4 * j) * vstride;
-      TG_HIP(hipMemcpyAsync(S, cur, bytes, hipMemcpyDeviceToDevice, s));
       for (int stg = 0; stg < 4; ++stg) {
         StageArgs<R> a{};
-        a.u0 = cur;
+        a.u0 = S;
         a.us = S + size_t(stg) * vstride;
-        a.unext = stg == 3 ? cur : S + size_t(stg + 1) * vstride;
+        a.unext = stg < 3 ? S + size_t(stg + 1) * vstride : (j + 1 < len ? S + size_t(4) * vstride : cur);
         a.uacc = w.Acc;
         a.dt = R(dt); a.nu = R(nu); a.ks = R(p->ks); a.stage = stg;
         a.cpz = kzp<R, N>();
@@ -480,6 +483,8 @@ struct Ops {
   }
   // Forward sweep: nsteps RK4 steps from x0 with a checkpoint every `seg`
   // steps, J = E(T) (objective 0) or Z(T) (1), and the terminal cotangent.
+  // The stage states (and kept inverse-y outputs) of the last segment go to
+  // the first stage buffer and those of the segment before it to the second.
   static int adj_forward(const tg_plan* p, const cpx<R>* x0, int nsteps, double dt, double nu, int objective,
                          int seg, double* J, void* ws, void* aws, hipStream_t s) {
     AdjLayout L = adj_layout(p, nsteps, seg, aws);
@@ -490,9 +495,12 @@ struct Ops {
     for (int c = 0; c < L.nck; ++c) {
       TG_HIP(hipMemcpyAsync(L.ck + c * L.vstride, L.cur, bytes, hipMemcpyDeviceToDevice, s));
       const int len = std::min(seg, nsteps - c * seg);
-      if (c < L.nck - 1) TG_RET(rk4_padded(p, L.cur, nullptr, len, dt, nu, 0.0, -1.0, ws, s));
-      else TG_RET(steps_keeping_stages(p, L.cur, L.st, L.vstride, len, dt, nu, ws, s, L.slots,
-                                       L.sstride));  // the reverse sweep starts here
+      if (c < L.nck - 2) TG_RET(rk4_padded(p, L.cur, nullptr, len, dt, nu, 0.0, -1.0, ws, s));
+      else if (c == L.nck - 2)  // kept in the second stage buffer: the reverse sweep never recomputes it
+        TG_RET(steps_keeping_stages(p, L.cur, L.st2, L.vstride, len, dt, nu, ws, s, L.slots2, L.sstride));
+      else
+        TG_RET(steps_keeping_stages(p, L.cur, L.st, L.vstride, len, dt, nu, ws, s, L.slots,
+                                    L.sstride));  // the reverse sweep starts here
     }
     // diagnostics read the dense layout: a copy of the final state in abuf
     TG_HIP(hipMemcpy2DAsync(L.abuf, KZ * e, L.cur, KZP * e, KZ * e, rows, hipMemcpyDeviceToDevice, s));
@@ -520,7 +528,8 @@ struct Ops {
     const R h = R(dt);
     const R cl[4] = {h / R(6), h / R(3), h / R(3), h / R(6)};  // weight of lamp in a_s (s = 1..4)
     const R cm[4] = {R(0), h / R(2), h / R(2), h};              // weight of mu_{s+1} in a_s
-    // The last segment's stages are kept by the forward sweep (buffer 0).
+    // The stages of the last segment are kept by the forward sweep in buffer
+    // 0, those of the segment before it in buffer 1.
     // Segment c - 1 is recomputed on the private stream into the other
     // buffer while segment c is reversed on `s`; events order the buffer
     // reuse both ways.
@@ -541,8 +550,9 @@ struct Ops {
       const bool even = (L.nck - 1 - c) % 2 == 0;
       cpx<R>* stc = even ? L.st : L.st2;
       cpx<R>* slc = even ? L.slots : L.slots2;
-      if (c < L.nck - 1) TG_HIP(hipStreamWaitEvent(s, ev_rec, 0));  // segment c recomputed
-      if (c > 0) {
+      // segment c recomputed (the last two were kept by the forward sweep)
+      if (c < L.nck - 2) TG_HIP(hipStreamWaitEvent(s, ev_rec, 0));
+      if (c > 0 && c - 1 != L.nck - 2) {
         // recompute segment c - 1 into the buffer segment c + 1 used
         const bool evn = (L.nck - c) % 2 == 0;
         cpx<R>* stn = evn ? L.st : L.st2;

@@ -787,9 +787,14 @@ template <> struct ZWPlan<256, 16> { using type = Radices<16, 16>; };
 template <> struct ZWPlan<256, 8> { using type = Radices<8, 8, 4>; };
 template <> struct ZWPlan<128, 16> { using type = Radices<16, 8>; };
 template <> struct ZWPlan<128, 8> { using type = Radices<8, 4, 4>; };
+template <> struct ZWPlan<256, 4> { using type = Radices<4, 4, 4, 4>; };
+template <> struct ZWPlan<128, 4> { using type = Radices<4, 4, 4, 2>; };
+template <> struct ZWPlan<64, 4> { using type = Radices<4, 4, 4>; };
+template <> struct ZWPlan<32, 4> { using type = Radices<4, 4, 2>; };
 template <> struct ZWPlan<64, 16> { using type = Radices<16, 4>; };
 template <> struct ZWPlan<64, 8> { using type = Radices<8, 8>; };
 template <> struct ZWPlan<32, 8> { using type = Radices<8, 4>; };
+template <> struct ZWPlan<32, 16> { using type = Radices<16, 2>; };
 template <> struct ZWPlan<96, 12> { using type = Radices<12, 4, 2>; };
 template <> struct ZWPlan<48, 12> { using type = Radices<12, 4>; };
 
@@ -1215,13 +1220,18 @@ void k_zw_mhd(const cpx<R>* __restrict__ in, cpx<R>* __restrict__ out, const cpx
   B::run(sm, blockIdx.x, in, out, tw, lorentz);
 }
 
-// Register-resident adjoint z pass: 8 points per lane (five sequences per
-// line), two wavefronts per SIMD.
+// Register-resident adjoint z pass (five sequences per line): adj_ze points
+// per lane, eight in FP64 (160 registers of line data, two wavefronts per
+// SIMD; four points gave four wavefronts and one more exchange stage and ran
+// slower) and sixteen in FP32 (one exchange stage fewer than eight).
+template <typename R>
+constexpr int adj_ze() { return sizeof(R) == 8 ? 8 : 16; }
+
 template <typename R, int N, int WPB>
 __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(2)))
 void k_zw_adj(const cpx<R>* __restrict__ in, const cpx<R>* __restrict__ qin, cpx<R>* __restrict__ out,
               const cpx<R>* __restrict__ tw) {
-  using B = ZWAdjBody<R, N, 8, WPB>;
+  using B = ZWAdjBody<R, N, adj_ze<R>(), WPB>;
   __shared__ cpx<R> sm[B::SMEM];
   B::run(sm, blockIdx.x, in, qin, out, tw);
 }
