@@ -17,6 +17,7 @@ import math
 
 import torch
 
+from ._lib import supported_sizes
 from .solver import get_plan
 
 OBJECTIVES = {"energy": 0, "enstrophy": 1}
@@ -67,7 +68,7 @@ def final_objective(u_hat0, nu, nsteps, dt, objective="enstrophy", seg=None, L=2
     dtype = torch.float64 if u_hat0.dtype == torch.complex128 else torch.float32
     M = (u_hat0.shape[-1] - 1)
     N = None
-    for n in (4, 8, 16, 32, 48, 64, 96, 128, 256):
+    for n in supported_sizes():
         if n // 3 == M and 2 * M + 1 == u_hat0.shape[-2]:
             N = n
             break

@@ -12,7 +12,7 @@
 
 // Grid sizes with compiled plans: X(n) for each.  The build compiles
 // tg_inst.hip once per (precision, size) of this list.
-#define TG_GRID_SIZES(X) X(4) X(8) X(16) X(32) X(48) X(64) X(96) X(128) X(256)
+#define TG_GRID_SIZES(X) X(4) X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512)
 
 namespace tg {
 

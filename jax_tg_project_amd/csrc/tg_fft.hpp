@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 
 #include <type_traits>
+#include <utility>
 
 namespace tg {
 
@@ -217,32 +218,52 @@ __device__ __forceinline__ void dft(cpx<R>* v) {
 template <int... Rs>
 struct Radices {};
 
+// The default plan is greedy: at each stage the largest radix of
+// {16, 12, 8, 4, 3, 2} that divides both E and what is left of N.  A size
+// with no such factorisation (a prime factor above 3, or a factor 3 that E
+// lacks) fails the static_assert.
+struct RadixList {
+  int r[24];
+  int n;
+};
+__host__ __device__ constexpr RadixList greedy_radices(int N, int E) {
+  RadixList a{};
+  const int cand[6] = {16, 12, 8, 4, 3, 2};
+  int rem = N;
+  while (rem > 1) {
+    int pick = 0;
+    for (int c : cand)
+      if (E % c == 0 && rem % c == 0) {
+        pick = c;
+        break;
+      }
+    if (pick == 0 || a.n == 24) {
+      a.n = -1;
+      return a;
+    }
+    a.r[a.n++] = pick;
+    rem /= pick;
+  }
+  return a;
+}
+
 template <int N, int E>
-struct PlanFor;
-#define TG_PLAN(N_, E_, ...) \
-  template <> struct PlanFor<N_, E_> { using type = Radices<__VA_ARGS__>; };
-TG_PLAN(4, 2, 2, 2)
-TG_PLAN(4, 4, 4)
-TG_PLAN(8, 4, 4, 2)
-TG_PLAN(8, 8, 8)
-TG_PLAN(16, 4, 4, 4)
-TG_PLAN(16, 8, 8, 2)
-TG_PLAN(16, 16, 16)
-TG_PLAN(32, 4, 4, 4, 2)
-TG_PLAN(32, 8, 8, 4)
-TG_PLAN(32, 16, 16, 2)
-TG_PLAN(48, 12, 12, 4)
-TG_PLAN(48, 24, 8, 3, 2)
-TG_PLAN(64, 4, 4, 4, 4)
-TG_PLAN(64, 8, 8, 8)
-TG_PLAN(64, 16, 16, 4)
-TG_PLAN(96, 12, 12, 4, 2)
-TG_PLAN(96, 24, 8, 4, 3)
-TG_PLAN(128, 8, 8, 8, 2)
-TG_PLAN(128, 16, 16, 8)
-TG_PLAN(256, 8, 8, 8, 4)
-TG_PLAN(256, 16, 16, 16)
-#undef TG_PLAN
+struct GreedyPlan {
+  static constexpr RadixList A = greedy_radices(N, E);
+  static_assert(A.n > 0, "no radix plan: N must factor into radices of E drawn from {16, 12, 8, 4, 3, 2}");
+  template <size_t... I>
+  static Radices<A.r[I]...> conv(std::index_sequence<I...>);
+  using type = decltype(conv(std::make_index_sequence<size_t(A.n > 0 ? A.n : 0)>{}));
+};
+
+template <int N, int E>
+struct PlanFor {
+  using type = typename GreedyPlan<N, E>::type;
+};
+// The 24-point plans of N = 48 and 96 (their LDS z pass) keep the radix
+// order they were tuned with instead of the greedy 12 x 4 / 12 x 8.
+template <> struct PlanFor<48, 24> { using type = Radices<8, 3, 2>; };
+template <> struct PlanFor<96, 24> { using type = Radices<8, 4, 3>; };
 
 // ---------------------------------------------------------------- global I/O
 // Stage with optional direct I/O.  SRC: 0 = LDS, 1 = functor (no LDS read, so

@@ -135,16 +135,16 @@ struct Launch {
   }
   // RHS modes take the tuned variant; edge modes (physical I/O, max |div|)
   // always run variant 0.  The hydro z pass is the register-resident kernel:
-  // N >= 128 variants 0 / 1 hold 16 / 8 points per lane (variant 2 is the
-  // LDS-resident kernel); smaller grids hold 8 (12 for N = 48, 96) points
-  // per lane, one wavefront per block.
+  // N >= 128 variants 0 / 1 hold ZwDefault E / E_ALT points per lane
+  // (variant 2 is the LDS-resident kernel); smaller grids hold ZwDefault E
+  // points per lane, one wavefront per block, or (E = 0) run the LDS kernel.
   static int z(int mode, int var, const cpx<R>* in, cpx<R>* out, const R* pin, R* pout, const cpx<R>* tw, R lor,
                hipStream_t s) {
     switch (mode) {
       case ZM_HYDRO:
         if constexpr (N >= 128) {
-          if (var == 0) return zw<16, 4>(in, out, tw, s);
-          if (var == 1) return zw<8, 4>(in, out, tw, s);
+          if (var == 0) return zw<ZwDefault<N>::E, 4>(in, out, tw, s);
+          if (var == 1) return zw<ZwDefault<N>::E_ALT, 4>(in, out, tw, s);
           return z_<ZM_HYDRO, 2>(in, out, pin, pout, tw, lor, s);
         } else if constexpr (ZwDefault<N>::E > 0) {
           return zw<ZwDefault<N>::E, ZwDefault<N>::WPB>(in, out, tw, s);
@@ -191,10 +191,10 @@ struct Launch {
     }
   }
   static int nmax() { return (N / CfgZM<R, N, ZM_MAXABS, 0>::LZ) * N; }
-  // RK4 stage variant 1 of N >= 128 (hydro and MHD) is the register-epilogue
-  // kernel; so is every adjoint launch on those grids.
+  // RK4 stage variant 1 of the power-of-two grids N >= 128 (hydro and MHD)
+  // is the register-epilogue kernel; so is every adjoint launch on those grids.
   static int fwd_x(int mode, int var, const cpx<R>* in, const StageArgs<R>& a, const cpx<R>* tw, hipStream_t s) {
-    if constexpr (N >= 128) {
+    if constexpr (use_xr<N>()) {
       if (mode == FM_ADJ) {
         using C = CfgXR<R, N>;
         hipLaunchKernelGGL((k_fwd_xr_adj<R, N>), dim3(cdiv(G::KZ, C::L), G::K), dim3(C::NT), 0, s, in, a, tw);
@@ -230,7 +230,7 @@ struct Launch {
   }
   // blocks of the adjoint forward-x pass (slots of the nu-gradient partials)
   static int adj_x_blocks() {
-    if constexpr (N >= 128) return cdiv(G::KZ, CfgXR<R, N>::L) * G::K;
+    if constexpr (use_xr<N>()) return cdiv(G::KZ, CfgXR<R, N>::L) * G::K;
     else return cdiv(G::KZ, CfgXF<R, N, 0>::LINES / 6) * G::K;
   }
   static int diag(const cpx<R>* su, const cpx<R>* sb, double* part, int nblk, double ks, hipStream_t s) {

@@ -47,39 +47,78 @@ constexpr bool nt_mid() { return N >= 256; }
 
 // Launch shapes.  Grids of N >= 128 compile NVAR tile variants per pass;
 // the plan picks one per pass at run time (see the autotuner); smaller grids
-// compile variant 0 only.
+// compile variant 0 only.  Every shape is a function of N: the points per
+// lane E divide N with a radix plan (N = 2^a or 3 * 2^a), a block's LDS tile
+// stays within the 160 KiB of a CU and its thread count within 1024.
 constexpr int NVAR = 3;
 template <int N>
 constexpr int num_variants() { return N >= 128 ? NVAR : 1; }
 
+template <typename R>
+constexpr bool lds_fits(long elems) { return elems * long(2 * sizeof(R)) <= 160L * 1024; }
+
+// points per lane of the line passes: 12 when 3 divides N, else 16 (N >= 128),
+// 8, or 4 (N = 4)
 template <int N>
-constexpr int base_e() { return (N == 48 || N == 96) ? 12 : (N >= 128 ? 16 : (N == 4 ? 4 : 8)); }
+constexpr int base_e() { return N % 3 == 0 ? 12 : (N >= 128 ? 16 : (N == 4 ? 4 : 8)); }
+// the fewer-points alternative of variant 1 (8, or 12 when 3 divides N)
+template <int N>
+constexpr int alt_e() { return N % 3 == 0 ? 12 : 8; }
 
 // X/Y passes: L columns of kz per block, E points per thread.
-//   V0: 16 columns x 16 points   V1: one 128-byte line of columns (8 FP64,
-//   16 FP32) x 8 points   V2: 32 columns x 16 points
+//   V0: 16 columns x base_e points   V1: one 128-byte line of columns (8
+//   FP64, 16 FP32) x alt_e points   V2: 32 columns x base_e points
+// Grids above 256 would double the block: V0 halves L while the block
+// exceeds 256 threads, V1 while it exceeds 512, and V2 takes 16 columns x
+// 2 base_e points when 32 columns exceed LDS or 512 threads.
+template <int N, int E>
+constexpr int xy_cap_l(int l, int cap) {
+  while (l > 8 && l * N / E > cap) l /= 2;
+  return l;
+}
+template <typename R, int N>
+constexpr bool xy_wide() { return lds_fits<R>(long(N) * 33) && 32 * N / base_e<N>() <= 512; }
 template <typename R, int N, int V>
 struct CfgXY {
-  static constexpr int E = (V == 1) ? 8 : base_e<N>();
-  static constexpr int L = V == 1 ? (sizeof(R) == 4 ? 16 : 8) : (V == 2 ? 32 : 16);
+  static constexpr int E = V == 1 ? alt_e<N>() : (V == 2 && !xy_wide<R, N>()) ? 2 * base_e<N>() : base_e<N>();
+  static constexpr int L = V == 1 ? xy_cap_l<N, E>(sizeof(R) == 4 ? 16 : 8, 512)
+                         : V == 2 ? (xy_wide<R, N>() ? 32 : 16) : xy_cap_l<N, E>(16, 256);
   static constexpr int NT = L * N / E;
+  static_assert(NT <= 1024 && lds_fits<R>(long(N) * (L + 1)), "x/y tile exceeds a workgroup");
 };
 // Z pass: NB = 3 x (hydro lines per block); E points per thread for the
 // inverse, E/2 for the forward (half as many sequences).
-//   V0: NB = 12 (N >= 256) or 24    V1: twice V0    V2: half of V0
+//   V0: NB = 12 (N >= 256 or N not a multiple of 8) or 24    V1: twice V0
+//   where LDS holds it    V2: half of V0
+template <int N>
+constexpr int z_nb0() { return (N >= 256 || N % 8 != 0) ? 12 : 24; }
+template <int N>
+constexpr int z_e() { return N % 3 == 0 ? (N >= 48 ? 24 : 12) : (N >= 128 ? 16 : (N == 4 ? 4 : 8)); }
 template <typename R, int N, int V>
 struct CfgZ {
-  static constexpr int NB0 = N >= 256 ? 12 : (N == 4 ? 12 : 24);
-  static constexpr int NB = V == 1 ? 2 * NB0 : (V == 2 ? NB0 / 2 : NB0);
-  static constexpr int E = (N == 48 || N == 96) ? 24 : (N >= 128 ? 16 : (N == 4 ? 4 : 8));
+  static constexpr int NB0 = z_nb0<N>();
+  // the widest block (NB sequences of N points) plus pad and twiddles
+  static constexpr int NB1 = lds_fits<R>(long(N) * (2 * NB0 + 3) + 1) ? 2 * NB0 : NB0;
+  static constexpr int NB = V == 1 ? NB1 : (V == 2 ? NB0 / 2 : NB0);
+  static constexpr int E = z_e<N>();
 };
 // Forward-x stage pass: LINES lines per block (3 fields x LINES/3 kz, or 6 x LINES/6).
-//   V0: 24 lines, E = 16   V1: 24 lines, E = 8   V2: 48 lines (FP32) / 12 lines (FP64), E = 16
+//   V0: 24 lines, E = base_e   V1: 24 lines, E = alt_e   V2: 48 lines (FP32)
+//   / 12 lines (FP64), E = base_e; each halved (down to 6) until the N x
+//   LINES tile fits LDS and the block 1024 threads (512 above N = 256)
+template <typename R, int N, int E>
+constexpr int xf_lines(int want) {
+  constexpr int cap = N > 256 ? 512 : 1024;
+  int l = want;
+  while (l > 6 && (!lds_fits<R>(long(N) * (l + 1)) || l * N / E > cap)) l /= 2;
+  return l;
+}
 template <typename R, int N, int V>
 struct CfgXF {
-  static constexpr int E = (V == 1) ? 8 : base_e<N>();
-  static constexpr int LINES = V == 2 ? (sizeof(R) == 4 ? 48 : 12) : 24;
+  static constexpr int E = (V == 1) ? alt_e<N>() : base_e<N>();
+  static constexpr int LINES = xf_lines<R, N, E>(V == 2 ? (sizeof(R) == 4 ? 48 : 12) : 24);
   static constexpr int NT = LINES * N / E;
+  static_assert(NT <= 1024 && LINES % 6 == 0 && lds_fits<R>(long(N) * (LINES + 1)), "x tile exceeds a workgroup");
 };
 
 enum XMode { XM_HYDRO = 0, XM_MHD = 1, XM_COPY3 = 2, XM_DIV = 3, XM_CURL3 = 4, XM_ADJ = 5, XM_ADJQ = 6,
@@ -613,7 +652,7 @@ struct ZBody {
     }
     return;
   }
-  if constexpr (MODE != ZM_PHYS_IN3) {
+  if constexpr (MODE != ZM_PHYS_IN3 && MODE != ZM_ADJ) {
     // ---- inverse: sequence s = line*SPL + p packs fields (fa, fb) as a + i b;
     // the first FFT stage reads the Hermitian-extended coefficients directly.
     auto ld = [&](int k, int s, int) -> cpx<R> {
@@ -782,21 +821,10 @@ __global__ __launch_bounds__((CfgZM<R, N, MODE, V>::NT)) void k_z(const cpx<R>* 
 // table is staged.  Forward sequences per line: n_x + i n_y (split by
 // Hermitian symmetry on the way out) and n_z + 0 i.
 template <int N, int E>
-struct ZWPlan;
-template <> struct ZWPlan<256, 16> { using type = Radices<16, 16>; };
-template <> struct ZWPlan<256, 8> { using type = Radices<8, 8, 4>; };
-template <> struct ZWPlan<128, 16> { using type = Radices<16, 8>; };
+struct ZWPlan {
+  using type = typename PlanFor<N, E>::type;
+};
 template <> struct ZWPlan<128, 8> { using type = Radices<8, 4, 4>; };
-template <> struct ZWPlan<256, 4> { using type = Radices<4, 4, 4, 4>; };
-template <> struct ZWPlan<128, 4> { using type = Radices<4, 4, 4, 2>; };
-template <> struct ZWPlan<64, 4> { using type = Radices<4, 4, 4>; };
-template <> struct ZWPlan<32, 4> { using type = Radices<4, 4, 2>; };
-template <> struct ZWPlan<64, 16> { using type = Radices<16, 4>; };
-template <> struct ZWPlan<64, 8> { using type = Radices<8, 8>; };
-template <> struct ZWPlan<32, 8> { using type = Radices<8, 4>; };
-template <> struct ZWPlan<32, 16> { using type = Radices<16, 2>; };
-template <> struct ZWPlan<96, 12> { using type = Radices<12, 4, 2>; };
-template <> struct ZWPlan<48, 12> { using type = Radices<12, 4>; };
 
 template <typename R, int N, int E_, int WPB>
 struct CfgZW {
@@ -1187,11 +1215,14 @@ struct ZWMhdBody {
 };
 
 // Default register z-pass shape of a grid size: points per lane and
-// wavefronts per block (E = 0: the grid uses the LDS-resident kernel).
+// wavefronts per block (E = 0: the grid uses the LDS-resident kernel).  A
+// wavefront holds whole lines, so N / E divides 64.
 template <int N>
 struct ZwDefault {
-  static constexpr int E = N >= 128 ? 16 : (N == 48 || N == 96) ? 12 : (N >= 32 ? 8 : 0);
+  static constexpr int E = N % 3 == 0 ? (N >= 48 ? 12 : 0) : (N >= 128 ? 16 : (N >= 32 ? 8 : 0));
   static constexpr int WPB = N >= 128 ? 4 : 1;
+  // the fewer-points register variant of N >= 128
+  static constexpr int E_ALT = N % 3 == 0 ? 6 : 8;
 };
 
 // Occupancy target (wavefronts per SIMD, i.e. a register budget of 512 /
@@ -1513,10 +1544,16 @@ __global__ __launch_bounds__((CfgXF<R, N, V>::NT)) void k_fwd_x(const cpx<R>* __
 // blocks share a CU and overlap their load, transform and update phases.
 template <typename R, int N>
 struct CfgXR {
-  static constexpr int E = 8;
-  static constexpr int L = sizeof(R) == 8 ? 8 : 16;  // one 128-byte line of kz per row
+  static constexpr int E = 8;  // power-of-two grids of N >= 128 only (use_xr)
+  // one 128-byte line of kz per row, halved in FP32 above 512 threads
+  static constexpr int L = (sizeof(R) == 8 || 16 * N / E <= 512) ? 128 / int(2 * sizeof(R)) : 8;
   static constexpr int NT = L * N / E;
 };
+
+// Grids whose RK4 stage variant 1 and adjoint forward-x pass run the
+// register-epilogue kernel: 8 points per lane need N a power of two.
+template <int N>
+constexpr bool use_xr() { return N >= 128 && (N & (N - 1)) == 0; }
 
 // Epilogues of the register forward-x pass: hydro or MHD RK4 stage, or the
 // adjoint bookkeeping of one stage VJP (the FM_ADJ epilogue of k_fwd_x).

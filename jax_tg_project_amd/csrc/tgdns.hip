@@ -116,7 +116,11 @@ int tg_plan_create(int n, int precision_bits, int mhd, double domain_length, tg_
     return fail(TG_ERR_ARG, "precision_bits must be 64 or 32");
   const void* ops = precision_bits == 64 ? static_cast<const void*>(ops_table<double>(n))
                                          : static_cast<const void*>(ops_table<float>(n));
-  if (!ops) return fail(TG_ERR_UNSUPPORTED, "grid size N=" + std::to_string(n) + " has no compiled plan");
+  if (!ops) {
+    std::string list;
+    for (int s : kSizes) list += " " + std::to_string(s);
+    return fail(TG_ERR_UNSUPPORTED, "grid size N=" + std::to_string(n) + " has no compiled plan (compiled:" + list + ")");
+  }
   if (!(domain_length > 0.0)) return fail(TG_ERR_ARG, "domain_length must be positive");
   tg_plan* p = new tg_plan{};
   p->n = n;

@@ -240,7 +240,7 @@ def test_fp32_adjoint_config_128_vs_fp64():
     assert abs(n32 - n64) <= 1e-4 * abs(n64)
 
 
-@pytest.mark.parametrize("N", [64, 96, 128, 256])
+@pytest.mark.parametrize("N", [24, 64, 96, 128, 192, 256])
 def test_vjp_matches_composed_transforms(N):
     """The fused VJP against the same formula composed from the separately
     verified transforms (tg_inverse / tg_forward) and torch elementwise
