@@ -46,6 +46,7 @@ extern "C" {
 #define TG_INFO_WS_BYTES 4
 #define TG_INFO_PRECISION 5
 #define TG_INFO_MHD 6
+#define TG_INFO_GENERIC 7 /* 1: N has no compiled kernels; the plan runs the direct-sum path */
 #define TG_INFO_COUNT 8
 
 typedef struct tg_plan tg_plan;
@@ -54,12 +55,17 @@ typedef struct tg_plan tg_plan;
  * TG_DIAG_COUNT values, TG_OP_GRAD. */
 int tg_version(void);
 
-/* Grid sizes with compiled plans, written to out[0..cap); returns the count. */
+/* Grid sizes with compiled plans, written to out[0..cap); returns the count.
+ * Plans of every other N in [4, 2048] run a direct-sum path (O(N^4) per
+ * transform, the reference's advective nonlinear term; no adjoint, no
+ * ensembles, no tg_diagnostics_noalloc). */
 int tg_supported_sizes(int* out, int cap);
 
 /* Create a plan for an N^3 grid of side `domain_length` (2*pi in every
- * reference configuration).  precision_bits: 64 or 32.  mhd: 0 = hydro
- * (velocity only), 1 = MHD (velocity + magnetic field).
+ * reference configuration), 4 <= N <= 2048 (the reference accepts any N >= 4;
+ * sizes outside tg_supported_sizes run the direct-sum path).
+ * precision_bits: 64 or 32.  mhd: 0 = hydro (velocity only), 1 = MHD
+ * (velocity + magnetic field).
  * Replaces the reference's per-call setup: make_wavenumbers /
  * make_dealias_mask (jax/taylor_green_jax.py:55-67) and
  * SpectralGrid.__init__ (cuda/taylor_green_cuda/grid.py:5-13). */

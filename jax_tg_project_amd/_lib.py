@@ -15,7 +15,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("TGDNS_LIB") or os.path.join(_HERE, "libtgdns.so")
 
 TG_INFO_N, TG_INFO_K, TG_INFO_KZ, TG_INFO_COMPACT_ELEMS, TG_INFO_WS_BYTES = 0, 1, 2, 3, 4
-TG_INFO_PRECISION, TG_INFO_MHD, TG_INFO_COUNT = 5, 6, 8
+TG_INFO_PRECISION, TG_INFO_MHD, TG_INFO_GENERIC, TG_INFO_COUNT = 5, 6, 7, 8
 TG_OP_FIELD, TG_OP_CURL, TG_OP_DIV, TG_OP_GRAD = 0, 1, 2, 3
 TG_DIAG_COUNT = 11
 
@@ -174,6 +174,8 @@ class Plan:
         self.KZ = int(info[TG_INFO_KZ])
         self.M = self.n // 3
         self.ws_bytes = int(info[TG_INFO_WS_BYTES])
+        # no compiled kernels for n: the direct-sum path (no adjoint, no ensembles)
+        self.generic = bool(info[TG_INFO_GENERIC])
         self.ws = torch.empty(self.ws_bytes, dtype=torch.uint8, device=self.device)
         self.spec_shape = (3, self.K, self.K, self.KZ)
         self.phys_shape = (3, self.n, self.n, self.n)

@@ -3,7 +3,8 @@
     python -m jax_tg_project_amd.build
 
 The kernels of each (precision, grid size) are one translation unit
-(csrc/tg_inst.hip compiled with -DTG_INST_PREC / -DTG_INST_N); the C ABI is
+(csrc/tg_inst.hip compiled with -DTG_INST_PREC / -DTG_INST_N); the
+direct-sum path of every other size is csrc/tg_generic.hip; the C ABI is
 csrc/tgdns.hip.  The objects compile in parallel and link into one shared
 library; csrc/tg_cli.cpp links against it into the native command-line
 solver taylor_green_hip.  Every call recompiles everything.
@@ -65,7 +66,8 @@ def build(verbose=True, out=OUT):
     cc = hipcc()
     tmpdir = tempfile.mkdtemp(prefix="tgdns_build_")
     try:
-        units = [(os.path.join(CSRC, "tgdns.hip"), os.path.join(tmpdir, "tgdns.o"), [])]
+        units = [(os.path.join(CSRC, "tgdns.hip"), os.path.join(tmpdir, "tgdns.o"), []),
+                 (os.path.join(CSRC, "tg_generic.hip"), os.path.join(tmpdir, "tg_generic.o"), [])]
         # largest grids first: they take longest
         for n in sorted(grid_sizes(), reverse=True):
             for prec in (64, 32):

@@ -52,8 +52,10 @@ struct tg_plan {
   void* tw;           // N complex twiddles exp(-2 pi i j / N) in plan precision
   size_t cb;          // bytes per complex element
   size_t compact;     // complex elements per compact component
-  // workspace layout (byte offsets)
+  // workspace layout (byte offsets); offP, offC only for generic plans
   size_t offA, offB, offU0, offU1, offAcc, offPart, offMax, offOut, wsBytes;
+  size_t offP = 0, offC = 0;
+  bool generic = false;  // no compiled kernels for n: direct-sum transforms (tg_generic.hip)
   int nDiagBlk, nMax;
   int var[TG_NPASS] = {0, 0, 0, 0, 0};  // tile variant per pass
   // optional per-pass timing: pairs of events around each pass launch

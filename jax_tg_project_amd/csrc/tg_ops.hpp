@@ -234,7 +234,7 @@ struct Launch {
     else return cdiv(G::KZ, CfgXF<R, N, 0>::LINES / 6) * G::K;
   }
   static int diag(const cpx<R>* su, const cpx<R>* sb, double* part, int nblk, double ks, hipStream_t s) {
-    hipLaunchKernelGGL((k_diag<R, N>), dim3(nblk), dim3(DIAG_NT), 0, s, su, sb, part, ks);
+    hipLaunchKernelGGL((k_diag<R>), dim3(nblk), dim3(DIAG_NT), 0, s, su, sb, part, ks, G::M);
     TG_LAUNCHED();
     return TG_OK;
   }

@@ -1817,19 +1817,22 @@ struct Neu {
   __device__ double val() const { return s + c; }
 };
 
-template <typename R, int N>
+// The grid enters through M = N/3 only (K = 2M+1, KZ = M+1), so one
+// instantiation per precision serves every grid size.
+template <typename R>
 __global__ __launch_bounds__(DIAG_NT) void k_diag(const cpx<R>* __restrict__ su,
                                                   const cpx<R>* __restrict__ sb,
-                                                  double* __restrict__ partial, double ks) {
-  using G = Geo<N>;
-  const size_t cs = size_t(G::K) * G::K * G::KZ;
+                                                  double* __restrict__ partial, double ks, int M) {
+  const int K = 2 * M + 1, KZ = M + 1;
+  const size_t cs = size_t(K) * K * KZ;
+  auto kint = [&](int i) { return i <= M ? i : i - K; };
   Neu acc[NDIAG];
   for (size_t pt = blockIdx.x * size_t(DIAG_NT) + threadIdx.x; pt < cs; pt += size_t(gridDim.x) * DIAG_NT) {
-    const int iz = int(pt % G::KZ);
-    const int iy = int((pt / G::KZ) % G::K);
-    const int ix = int(pt / (size_t(G::KZ) * G::K));
+    const int iz = int(pt % KZ);
+    const int iy = int((pt / KZ) % K);
+    const int ix = int(pt / (size_t(KZ) * K));
     const double w = iz == 0 ? 1.0 : 2.0;
-    const double kx = ks * G::kint(ix), ky = ks * G::kint(iy), kz = ks * iz;
+    const double kx = ks * kint(ix), ky = ks * kint(iy), kz = ks * iz;
     const double k2 = kx * kx + ky * ky + kz * kz;
     double ur[3], ui[3];
 #pragma unroll

@@ -41,12 +41,18 @@ int fail(int code, const std::string& msg) {
   extern const tg::EnsTable<float> tg_ens_32_##n;
 TG_GRID_SIZES(TG_DECL_TABLES)
 #undef TG_DECL_TABLES
+// Every other size: the direct-sum path of tg_generic.hip.
+extern const tg::OpsTable<double> tg_ops_64_generic;
+extern const tg::OpsTable<float> tg_ops_32_generic;
 
 namespace {
 
 #define TG_LIST_SIZE(n) n,
 constexpr int kSizes[] = {TG_GRID_SIZES(TG_LIST_SIZE)};
 #undef TG_LIST_SIZE
+// Largest grid a plan accepts (the generic path indexes N^3 points in 64 bits
+// and phases k n in 64 bits; the bound keeps a workspace request sane).
+constexpr int kMaxGeneric = 2048;
 
 template <typename R>
 const OpsTable<R>* ops_table(int n);
@@ -114,19 +120,22 @@ int tg_plan_create(int n, int precision_bits, int mhd, double domain_length, tg_
   *out = nullptr;
   if (precision_bits != 64 && precision_bits != 32)
     return fail(TG_ERR_ARG, "precision_bits must be 64 or 32");
+  if (n < 4 || n > kMaxGeneric)
+    return fail(TG_ERR_UNSUPPORTED, "grid size N=" + std::to_string(n) + " outside [4, " +
+                                        std::to_string(kMaxGeneric) + "]");
   const void* ops = precision_bits == 64 ? static_cast<const void*>(ops_table<double>(n))
                                          : static_cast<const void*>(ops_table<float>(n));
-  if (!ops) {
-    std::string list;
-    for (int s : kSizes) list += " " + std::to_string(s);
-    return fail(TG_ERR_UNSUPPORTED, "grid size N=" + std::to_string(n) + " has no compiled plan (compiled:" + list + ")");
-  }
+  const bool generic = ops == nullptr;
+  if (generic)
+    ops = precision_bits == 64 ? static_cast<const void*>(&tg_ops_64_generic)
+                               : static_cast<const void*>(&tg_ops_32_generic);
   if (!(domain_length > 0.0)) return fail(TG_ERR_ARG, "domain_length must be positive");
   tg_plan* p = new tg_plan{};
   p->n = n;
   p->prec = precision_bits;
   p->mhd = mhd ? 1 : 0;
   p->ops = ops;
+  p->generic = generic;
   p->M = n / 3;
   p->K = 2 * p->M + 1;
   p->KZ = p->M + 1;
@@ -136,22 +145,41 @@ int tg_plan_create(int n, int precision_bits, int mhd, double domain_length, tg_
   p->cb = precision_bits == 64 ? 16 : 8;
   p->compact = size_t(p->K) * p->K * p->KZ;
   const size_t N = n;
-  const size_t fi = p->mhd ? 12 : 9;  // inverse fields: hydro RHS 6, adjoint 9, MHD 12
-  const size_t fo = 6;                 // z-pass outputs: hydro 3, adjoint and MHD 6
-  // A holds the inverse-x output (fi x N K KZ) and later the z output (fo x N N KZ)
-  const size_t kzpad = precision_bits == 64 ? size_t(ops_table<double>(n)->kzp) : size_t(ops_table<float>(n)->kzp);
-  const size_t sA = std::max(fi * N * p->K, fo * N * N) * kzpad * p->cb;
-  const size_t sB = fi * N * N * kzpad * p->cb;
-  // padded state vectors (row pitch kzpad) for the RK4 loop
-  const size_t sU = (p->mhd ? 2 : 1) * 3 * size_t(p->K) * p->K * kzpad * p->cb;
   p->nDiagBlk = 512;
-  p->nMax = precision_bits == 64 ? ops_table<double>(n)->nmax : ops_table<float>(n)->nmax;
   size_t o = 0;
-  p->offA = o; o += align256(sA);
-  p->offB = o; o += align256(sB);
-  p->offU0 = o; o += align256(sU);
-  p->offU1 = o; o += align256(sU);
-  p->offAcc = o; o += align256(sU);
+  if (generic) {
+    // x-pass and y-pass buffers of one field, the physical fields of the
+    // RHS (hydro u, T, W; MHD also b, j), compact temporaries (F[u.grad u];
+    // MHD also F[j x b], F[u x b]) and the three RK4 vectors (rhs, stage
+    // input, accumulator) in the dense compact layout
+    const size_t sA = N * p->K * p->KZ * p->cb, sB = N * N * p->KZ * p->cb;
+    const size_t sP = (p->mhd ? 13 : 7) * N * N * N * (p->cb / 2);
+    const size_t sC = (p->mhd ? 9 : 3) * p->compact * p->cb;
+    const size_t sU = (p->mhd ? 6 : 3) * p->compact * p->cb;
+    p->nMax = precision_bits == 64 ? tg_ops_64_generic.nmax : tg_ops_32_generic.nmax;
+    p->offA = o; o += align256(sA);
+    p->offB = o; o += align256(sB);
+    p->offP = o; o += align256(sP);
+    p->offC = o; o += align256(sC);
+    p->offU0 = o; o += align256(sU);
+    p->offU1 = o; o += align256(sU);
+    p->offAcc = o; o += align256(sU);
+  } else {
+    const size_t fi = p->mhd ? 12 : 9;  // inverse fields: hydro RHS 6, adjoint 9, MHD 12
+    const size_t fo = 6;                 // z-pass outputs: hydro 3, adjoint and MHD 6
+    // A holds the inverse-x output (fi x N K KZ) and later the z output (fo x N N KZ)
+    const size_t kzpad = precision_bits == 64 ? size_t(ops_table<double>(n)->kzp) : size_t(ops_table<float>(n)->kzp);
+    const size_t sA = std::max(fi * N * p->K, fo * N * N) * kzpad * p->cb;
+    const size_t sB = fi * N * N * kzpad * p->cb;
+    // padded state vectors (row pitch kzpad) for the RK4 loop
+    const size_t sU = (p->mhd ? 2 : 1) * 3 * size_t(p->K) * p->K * kzpad * p->cb;
+    p->nMax = precision_bits == 64 ? ops_table<double>(n)->nmax : ops_table<float>(n)->nmax;
+    p->offA = o; o += align256(sA);
+    p->offB = o; o += align256(sB);
+    p->offU0 = o; o += align256(sU);
+    p->offU1 = o; o += align256(sU);
+    p->offAcc = o; o += align256(sU);
+  }
   p->offPart = o; o += align256(size_t(p->nDiagBlk) * NDIAG * sizeof(double));
   p->offMax = o; o += align256(2 * size_t(p->nMax) * (p->cb / 2));
   p->offOut = o; o += align256((NDIAG + 2) * sizeof(double));
@@ -205,7 +233,7 @@ int tg_plan_query(const tg_plan* p, long long* info) {
   info[TG_INFO_WS_BYTES] = (long long)p->wsBytes;
   info[TG_INFO_PRECISION] = p->prec;
   info[TG_INFO_MHD] = p->mhd;
-  info[7] = 0;
+  info[TG_INFO_GENERIC] = p->generic ? 1 : 0;
   return TG_OK;
 }
 
@@ -282,6 +310,7 @@ int tg_vjp(tg_plan* p, const void* x_hat, const void* a_hat, double nu, void* ou
 int tg_adjoint_workspace(tg_plan* p, int nsteps, int seg, long long* bytes) {
   TG_RET(check_plan(p));
   if (nsteps < 1 || seg < 1 || !bytes) return fail(TG_ERR_ARG, "tg_adjoint_workspace: need nsteps, seg >= 1");
+  if (p->generic) return fail(TG_ERR_UNSUPPORTED, "adjoint: N=" + std::to_string(p->n) + " has no compiled plan");
   const size_t kzpad = p->prec == 64 ? size_t(ops_table<double>(p->n)->kzp) : size_t(ops_table<float>(p->n)->kzp);
   const size_t vb = align256(3 * size_t(p->K) * p->K * kzpad * p->cb);  // line-padded vector
   const size_t nck = (nsteps + seg - 1) / seg;
@@ -324,7 +353,7 @@ int tg_adjoint_backward(tg_plan* p, int nsteps, double dt, double nu, int seg, v
 
 int tg_plan_variants(tg_plan* p, const int* set, int* get, int* count) {
   TG_RET(check_plan(p));
-  const int nv = p->n >= 128 ? NVAR : 1;
+  const int nv = (!p->generic && p->n >= 128) ? NVAR : 1;
   if (count) *count = nv;
   if (set) {
     for (int i = 0; i < TG_NPASS; ++i)
@@ -721,6 +750,9 @@ int tg_diagnostics_noalloc(tg_plan* p, const void* u_hat, const void* b_hat, int
                            void* stream) {
   TG_RET(check_plan(p));
   if (!u_hat || !out_dev) return fail(TG_ERR_ARG, "tg_diagnostics_noalloc: NULL buffer");
+  if (p->generic)
+    return fail(TG_ERR_UNSUPPORTED, "tg_diagnostics_noalloc: N=" + std::to_string(p->n) +
+                                        " has no compiled plan; use tg_diagnostics");
   return with_table(p, [&](const auto& T, auto* rt) {
     using R = std::remove_pointer_t<decltype(rt)>;
     const size_t need = T.diag_scratch_bytes(p, with_divmax);

@@ -39,10 +39,12 @@ def test_abi_host_side_errors_without_gpu():
     assert lib.tg_version() >= 2
     assert _lib.supported_sizes() == (4, 8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512)
     h = ctypes.c_void_p()
-    assert lib.tg_plan_create(100, 64, 0, 2 * math.pi, ctypes.byref(h)) == 3  # TG_ERR_UNSUPPORTED
+    # N outside [4, 2048] is rejected before any device call
+    assert lib.tg_plan_create(3, 64, 0, 2 * math.pi, ctypes.byref(h)) == 3  # TG_ERR_UNSUPPORTED
+    assert lib.tg_plan_create(4096, 64, 0, 2 * math.pi, ctypes.byref(h)) == 3
     buf = ctypes.create_string_buffer(256)
     lib.tg_last_error(buf, 256)
-    assert b"N=100" in buf.value
+    assert b"N=4096" in buf.value
     assert lib.tg_plan_create(64, 16, 0, 2 * math.pi, ctypes.byref(h)) == 1  # TG_ERR_ARG
     assert lib.tg_plan_create(64, 64, 0, -1.0, ctypes.byref(h)) == 1
     assert lib.tg_rk4_steps(None, None, None, 1, 0.1, 0.1, 0.0, -1.0, None, None) == 1
