@@ -186,10 +186,16 @@ int tg_batch_destroy(tg_batch* batch);
 int tg_batch_query(const tg_batch* batch, long long* state_offsets, long long* ws_bytes);
 /* Record the state and workspace device pointers used by later steps. */
 int tg_batch_bind(tg_batch* batch, void* state, void* ws);
+/* Launch layout of a step.  mixed != 0 (the default when the batch holds
+ * any of N = 32, 48, 64, 96): one launch per pass covers every instance of
+ * those sizes, whose batched bodies share block shapes; mixed == 0: one
+ * launch per pass and size class.  Other sizes always launch per class. */
+int tg_batch_set_launch(tg_batch* batch, int mixed);
 /* nsteps classic RK4 steps of every instance with time step dt.  With
- * use_graph != 0 one step (4 stages x 5 batched launches per size class) is
- * captured once into a HIP graph (again when dt changes), with one parallel
- * branch per size class, and replayed nsteps times. */
+ * use_graph != 0 one step (4 stages x 5 batched launches per launch unit:
+ * the mixed group, or a size class) is captured once into a HIP graph
+ * (again when dt or the layout changes), with one parallel branch per unit,
+ * and replayed nsteps times. */
 int tg_batch_rk4_steps(tg_batch* batch, int nsteps, double dt, int use_graph, void* stream);
 
 /* ---- Adjoint (no counterpart in the reference, whose only gradient would be

@@ -60,6 +60,7 @@ _SIGS = {
     "tg_batch_query": ([_VP, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)],
                        ctypes.c_int),
     "tg_batch_bind": ([_VP, _VP, _VP], ctypes.c_int),
+    "tg_batch_set_launch": ([_VP, ctypes.c_int], ctypes.c_int),
     "tg_batch_rk4_steps": ([_VP, ctypes.c_int, ctypes.c_double, ctypes.c_int, _VP], ctypes.c_int),
     "tg_vjp": ([_VP, _VP, _VP, ctypes.c_double, _VP, _VP, _VP], ctypes.c_int),
     "tg_adjoint_workspace": ([_VP, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_longlong)], ctypes.c_int),

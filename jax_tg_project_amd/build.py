@@ -67,7 +67,8 @@ def build(verbose=True, out=OUT):
     tmpdir = tempfile.mkdtemp(prefix="tgdns_build_")
     try:
         units = [(os.path.join(CSRC, "tgdns.hip"), os.path.join(tmpdir, "tgdns.o"), []),
-                 (os.path.join(CSRC, "tg_generic.hip"), os.path.join(tmpdir, "tg_generic.o"), [])]
+                 (os.path.join(CSRC, "tg_generic.hip"), os.path.join(tmpdir, "tg_generic.o"), []),
+                 (os.path.join(CSRC, "tg_ens_mixed.hip"), os.path.join(tmpdir, "tg_ens_mixed.o"), [])]
         # largest grids first: they take longest
         for n in sorted(grid_sizes(), reverse=True):
             for prec in (64, 32):

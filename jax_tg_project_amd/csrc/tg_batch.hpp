@@ -151,6 +151,62 @@ __attribute__((amdgpu_waves_per_eu(EnsBlock<R, N, PASS>::WAVES))) void kb_pass(c
   ens_run<R, N, PASS>(sm, b - prefix[lo], inst[lo], dt, stage);
 }
 
+// Sizes whose batched x bodies (256 threads) and z bodies (one wavefront)
+// share their block shapes, so one launch per pass can cover instances of
+// all of them: a block finds its instance as in kb_pass and branches (block-
+// uniformly) to that instance's size body.  The y bodies differ in threads
+// (PPB planes x KZ x N/E); the mixed y launch takes the largest, and the
+// threads a smaller body does not use stay inactive through its transforms
+// while still reaching every barrier.
+#define TG_MIX_SIZES_LIST 96, 64, 48, 32
+constexpr int kMixSizes[] = {TG_MIX_SIZES_LIST};
+
+inline constexpr bool is_mix_size(int n) {
+  for (int s : kMixSizes)
+    if (s == n) return true;
+  return false;
+}
+
+template <typename R, int PASS>
+struct MixBlock {
+  template <int... Ns>
+  static constexpr int max_nt() { int m = 0; ((m = EnsBlock<R, Ns, PASS>::NT > m ? EnsBlock<R, Ns, PASS>::NT : m), ...); return m; }
+  template <int... Ns>
+  static constexpr int min_nt() { int m = 1 << 20; ((m = EnsBlock<R, Ns, PASS>::NT < m ? EnsBlock<R, Ns, PASS>::NT : m), ...); return m; }
+  template <int... Ns>
+  static constexpr int max_smem() { int m = 0; ((m = EnsBlock<R, Ns, PASS>::SMEM > m ? EnsBlock<R, Ns, PASS>::SMEM : m), ...); return m; }
+  template <int... Ns>
+  static constexpr int min_waves() { int m = 16; ((m = EnsBlock<R, Ns, PASS>::WAVES < m ? EnsBlock<R, Ns, PASS>::WAVES : m), ...); return m; }
+  static constexpr int NT = max_nt<TG_MIX_SIZES_LIST>();
+  static constexpr int SMEM = max_smem<TG_MIX_SIZES_LIST>();
+  static constexpr int WAVES = min_waves<TG_MIX_SIZES_LIST>();
+  // the x and z bodies index lanes by their own block shape: it must match
+  static_assert(PASS == EP_INV_Y || PASS == EP_FWD_Y || min_nt<TG_MIX_SIZES_LIST>() == NT,
+                "mixed x / z launches need one block shape");
+};
+
+template <typename R, int PASS>
+__global__ __launch_bounds__((MixBlock<R, PASS>::NT))
+__attribute__((amdgpu_waves_per_eu(MixBlock<R, PASS>::WAVES))) void kb_mixed(const EnsInst<R>* __restrict__ inst,
+                                                                      const int* __restrict__ prefix, int ninst,
+                                                                      R dt, int stage) {
+  __shared__ cpx<R> sm[MixBlock<R, PASS>::SMEM];
+  const int b = blockIdx.x;
+  int lo = 0, hi = ninst;  // prefix[lo] <= b < prefix[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (prefix[mid] <= b) lo = mid;
+    else hi = mid;
+  }
+  const int lb = b - prefix[lo];
+  switch (inst[lo].n) {
+    case 96: ens_run<R, 96, PASS>(sm, lb, inst[lo], dt, stage); break;
+    case 64: ens_run<R, 64, PASS>(sm, lb, inst[lo], dt, stage); break;
+    case 48: ens_run<R, 48, PASS>(sm, lb, inst[lo], dt, stage); break;
+    default: ens_run<R, 32, PASS>(sm, lb, inst[lo], dt, stage); break;
+  }
+}
+
 inline constexpr bool is_ens_size(int n) {
   for (int s : kEnsSizes)
     if (s == n) return true;
@@ -187,6 +243,39 @@ int ens_stage(const EnsInst<R>* inst, const int* dpre, const int* hpre, int cnt,
   TG_RET(ens_launch<R, N, EP_Z>(inst, dpre, hpre, cnt, st, dt, s));
   TG_RET(ens_launch<R, N, EP_FWD_Y>(inst, dpre, hpre, cnt, st, dt, s));
   return ens_launch<R, N, EP_FWD_X>(inst, dpre, hpre, cnt, st, dt, s);
+}
+
+// The stage of the mixed launches: `cnt` descriptors from `inst` (every one
+// of a kMixSizes size), dpre / hpre as for EnsTable::stage.
+template <typename R>
+struct EnsMixTable {
+  int (*stage)(const EnsInst<R>* inst, const int* dpre, const int* hpre, int cnt, int st, double dt,
+               hipStream_t s);
+};
+
+template <typename R, int PASS>
+int mix_launch(const EnsInst<R>* inst, const int* dpre, const int* hpre, int cnt, int st, double dt, hipStream_t s) {
+  const int nblk = hpre[PASS * (cnt + 1) + cnt];
+  hipLaunchKernelGGL((kb_mixed<R, PASS>), dim3(nblk), dim3(MixBlock<R, PASS>::NT), 0, s, inst,
+                     dpre + PASS * (cnt + 1), cnt, R(dt), st);
+  const hipError_t e = hipGetLastError();
+  return e == hipSuccess ? TG_OK : fail(TG_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+}
+
+template <typename R>
+int mix_stage(const EnsInst<R>* inst, const int* dpre, const int* hpre, int cnt, int st, double dt, hipStream_t s) {
+  TG_RET(mix_launch<R, EP_INV_X>(inst, dpre, hpre, cnt, st, dt, s));
+  TG_RET(mix_launch<R, EP_INV_Y>(inst, dpre, hpre, cnt, st, dt, s));
+  TG_RET(mix_launch<R, EP_Z>(inst, dpre, hpre, cnt, st, dt, s));
+  TG_RET(mix_launch<R, EP_FWD_Y>(inst, dpre, hpre, cnt, st, dt, s));
+  return mix_launch<R, EP_FWD_X>(inst, dpre, hpre, cnt, st, dt, s);
+}
+
+template <typename R>
+EnsMixTable<R> make_mix_table() {
+  EnsMixTable<R> t{};
+  t.stage = &mix_stage<R>;
+  return t;
 }
 
 template <typename R, int N>

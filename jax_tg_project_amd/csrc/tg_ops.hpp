@@ -76,7 +76,7 @@ struct Launch {
     return with_variant<N>(var, [&](auto vc) {
       constexpr int V = decltype(vc)::value;
       using C = CfgXY<R, N, V>;
-      dim3 grid(cdiv(G::KZ, C::L), G::K), blk(C::NT);
+      dim3 grid(cdiv(G::KZ, C::L), G::K, mode == XM_MHD5 ? 2 : 1), blk(C::NT);
       switch (mode) {
         case XM_HYDRO: hipLaunchKernelGGL((k_inv_x<R, N, XM_HYDRO, V>), grid, blk, 0, s, su, sb, out, tw, ks, al, cpz); break;
         case XM_MHD: hipLaunchKernelGGL((k_inv_x<R, N, XM_MHD, V>), grid, blk, 0, s, su, sb, out, tw, ks, al, cpz); break;
@@ -152,14 +152,13 @@ struct Launch {
           return z_<ZM_HYDRO, 0>(in, out, pin, pout, tw, lor, s);
         }
       case ZM_MHD:
-        // variant 0 of the power-of-two grids >= 32 is the register-resident kernel
+        // variant 0 of the power-of-two grids >= 32 is the register-resident
+        // kernel at 8 points per lane; variant 1 of those up to 256 the same
+        // at 4 points per lane (N / 4 lanes per line fit a wavefront)
         if constexpr (adj_reg_z<N>()) {
-          if (var == 0) {
-            constexpr int WPB = ZwDefault<N>::WPB;
-            using C = CfgZW<R, N, 8, WPB>;
-            hipLaunchKernelGGL((k_zw_mhd<R, N, WPB>), dim3(N * N / C::LINES), dim3(C::NT), 0, s, in, out, tw, lor);
-            TG_LAUNCHED();
-            return TG_OK;
+          if (var == 0) return zw_mhd<8>(in, out, tw, lor, s);
+          if constexpr (N <= 256 && num_variants<N>() > 1) {
+            if (var == 1) return zw_mhd<4>(in, out, tw, lor, s);
           }
         }
         return with_variant<N>(var, [&](auto vc) {
@@ -176,6 +175,15 @@ struct Launch {
         }
       default: return z_<ZM_PHYS_IN3, 0>(in, out, pin, pout, tw, lor, s);
     }
+  }
+  template <int E>
+  static int zw_mhd(const cpx<R>* in, cpx<R>* out, const cpx<R>* tw, R lor, hipStream_t s) {
+    constexpr int WPB = ZwDefault<N>::WPB;
+    using C = CfgZW<R, N, E, WPB>;
+    static_assert((N * N) % C::LINES == 0, "lines per block must divide N^2");
+    hipLaunchKernelGGL((k_zw_mhd<R, N, E, WPB>), dim3(N * N / C::LINES), dim3(C::NT), 0, s, in, out, tw, lor);
+    TG_LAUNCHED();
+    return TG_OK;
   }
   // Register-resident adjoint z pass: u, w from fields 0-5 of `in`, q from
   // fields 0-2 of `qin`.
@@ -203,7 +211,7 @@ struct Launch {
       }
       if ((mode == FM_HYDRO_STAGE || mode == FM_MHD_STAGE) && var == 1) {
         using C = CfgXR<R, N>;
-        const dim3 grid(cdiv(G::KZ, C::L), G::K), blk(C::NT);
+        const dim3 grid(cdiv(G::KZ, C::L), G::K, mode == FM_MHD_STAGE ? 2 : 1), blk(C::NT);
         if (mode == FM_HYDRO_STAGE) hipLaunchKernelGGL((k_fwd_xr<R, N, false>), grid, blk, 0, s, in, a, tw);
         else hipLaunchKernelGGL((k_fwd_xr<R, N, true>), grid, blk, 0, s, in, a, tw);
         TG_LAUNCHED();

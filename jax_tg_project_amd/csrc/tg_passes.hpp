@@ -355,9 +355,19 @@ void k_inv_x(const cpx<R>* __restrict__ su,
                                                               cpx<R>* __restrict__ out,
                                                               const cpx<R>* __restrict__ tw, R ks,
                                                               R alpha, int cpz) {
-  using B = XInvBody<R, N, MODE, V>;
-  __shared__ cpx<R> sm[B::SMEM];
-  B::run(sm, blockIdx.x, blockIdx.y, su, sb, out, tw, ks, alpha, cpz);
+  if constexpr (MODE == XM_MHD5) {
+    // the five fields of u (blockIdx.z = 0) and those of b (fields 5-9,
+    // blockIdx.z = 1) are independent: two blocks per tile instead of one
+    using B = XInvBody<R, N, XM_HYDRO5, V>;
+    __shared__ cpx<R> sm[B::SMEM];
+    const bool mag = blockIdx.z != 0;
+    B::run(sm, blockIdx.x, blockIdx.y, mag ? sb : su, nullptr,
+           out + (mag ? size_t(5) * N * Geo<N>::K * kzp<R, N>() : 0), tw, ks, alpha, cpz);
+  } else {
+    using B = XInvBody<R, N, MODE, V>;
+    __shared__ cpx<R> sm[B::SMEM];
+    B::run(sm, blockIdx.x, blockIdx.y, su, sb, out, tw, ks, alpha, cpz);
+  }
 }
 
 // ----------------------------------------------------------------- pass 2/4
@@ -1242,11 +1252,15 @@ void k_zw(const cpx<R>* __restrict__ in, cpx<R>* __restrict__ out, const cpx<R>*
   B::run(sm, blockIdx.x, in, out, tw);
 }
 
-// Register-resident MHD z pass: 8 points per lane (six sequences per line).
-template <typename R, int N, int WPB>
-__global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 8 ? 2 : 3)))
+// Register-resident MHD z pass: E points per lane (six sequences per line),
+// 8 (variant 0) or 4 (variant 1: half the line registers, more waves).
+template <typename R, int E>
+constexpr int zw_mhd_waves_per_eu() { return E <= 4 ? (sizeof(R) == 8 ? 3 : 4) : (sizeof(R) == 8 ? 2 : 3); }
+
+template <typename R, int N, int E, int WPB>
+__global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(zw_mhd_waves_per_eu<R, E>())))
 void k_zw_mhd(const cpx<R>* __restrict__ in, cpx<R>* __restrict__ out, const cpx<R>* __restrict__ tw, R lorentz) {
-  using B = ZWMhdBody<R, N, 8, WPB>;
+  using B = ZWMhdBody<R, N, E, WPB>;
   __shared__ cpx<R> sm[B::SMEM];
   B::run(sm, blockIdx.x, in, out, tw, lorentz);
 }
@@ -1565,7 +1579,9 @@ struct XRBody {
   using C = CfgXR<R, N>;
   static constexpr bool MHD = KIND == XR_MHD;
   static constexpr int SMEM = N * C::L + N;
-  static constexpr int FF = KIND == XR_HYDRO ? 3 : 6;  // fields: n (hydro); m, e (MHD); adjoint 6
+  // fields per block: n (hydro); m or e (MHD: blockIdx.z = 0 updates u from
+  // m, 1 updates b from e); adjoint 6
+  static constexpr int FF = KIND == XR_ADJ ? 6 : 3;
   __device__ __forceinline__ static void run(cpx<R>* sm, int bx, int by, const cpx<R>* __restrict__ in,
                                              const StageArgs<R>& a, const cpx<R>* __restrict__ tw) {
     constexpr int L = C::L, E = C::E;
@@ -1614,12 +1630,14 @@ struct XRBody {
       adjoint_epilogue(buf, w, colok, t, iy, iz, a);
       return;
     }
+    const bool mag = MHD && blockIdx.z != 0;
+    const int f0 = mag ? 3 : 0;
     cpx<R> o[FF][E];
 #pragma unroll
     for (int f = 0; f < FF; ++f) {
       if (f) __syncthreads();  // the previous field's last LDS reads are done
       auto ld = [&](int n, int, int) -> cpx<R> {
-        const cpx<R> v = gld<nt_mid<N>()>(&in[((size_t(f) * N + n) * G::K + iy) * kzp<R, N>() + iz]);
+        const cpx<R> v = gld<nt_mid<N>()>(&in[((size_t(f0 + f) * N + n) * G::K + iy) * kzp<R, N>() + iz]);
         return colok ? v : cpx<R>{R(0), R(0)};
       };
       auto st = [&](int, int, cpx<R> v, int slot) { o[f][slot] = v; };
@@ -1642,25 +1660,28 @@ struct XRBody {
         const R kx = a.ks * R(G::kint(ix));
         const R k2 = kx * kx + ky * ky + kz * kz;
         const R k2s = (k2 == R(0)) ? R(1) : k2;
-        // every load of the u half before its stores (see rk4_load)
-        cpx<R> su[3];
+        if (!mag) {
+          // every load of the u half before its stores (see rk4_load)
+          cpx<R> su[3];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) su[c] = a.us[c * cs + pt];
-        const RK4In<R> inu = rk4_load(a.stage, pt, cs, a.u0, a.uacc);
-        // momentum: r_u = P(n or m) - nu k2 u_s
-        project(kx, ky, kz, k2s, v);
-        cpx<R> ru[3];
-        const R nk = a.nu * k2;
+          for (int c = 0; c < 3; ++c) su[c] = a.us[c * cs + pt];
+          const RK4In<R> inu = rk4_load(a.stage, pt, cs, a.u0, a.uacc);
+          // momentum: r_u = P(n or m) - nu k2 u_s
+          project(kx, ky, kz, k2s, v);
+          cpx<R> ru[3];
+          const R nk = a.nu * k2;
 #pragma unroll
-        for (int c = 0; c < 3; ++c) ru[c] = {v[c].x - nk * su[c].x, v[c].y - nk * su[c].y};
-        rk4_store(a.stage, a.dt, pt, cs, ru, inu, a.unext, a.uacc);
+          for (int c = 0; c < 3; ++c) ru[c] = {v[c].x - nk * su[c].x, v[c].y - nk * su[c].y};
+          rk4_store(a.stage, a.dt, pt, cs, ru, inu, a.unext, a.uacc);
+        }
         if constexpr (MHD) {
+          if (!mag) continue;
           // induction: r_b = P(i k x e) - eta k2 b_s (the b half's loads, then its stores)
           cpx<R> sbv[3];
 #pragma unroll
           for (int c = 0; c < 3; ++c) sbv[c] = a.bs[c * cs + pt];
           const RK4In<R> inb = rk4_load(a.stage, pt, cs, a.b0, a.bacc);
-          const cpx<R> e0 = o[3][j * RL + m], e1 = o[4][j * RL + m], e2 = o[5][j * RL + m];
+          const cpx<R> e0 = v[0], e1 = v[1], e2 = v[2];
           cpx<R> cb[3];
           cb[0] = cmuli(cpx<R>{ky * e2.x - kz * e1.x, ky * e2.y - kz * e1.y});
           cb[1] = cmuli(cpx<R>{kz * e0.x - kx * e2.x, kz * e0.y - kx * e2.y});
@@ -1719,12 +1740,13 @@ struct XRBody {
   }
 };
 
-// Register budget: three waves per SIMD for the FP64 hydro kernel (its few
-// spills cost less than the lost occupancy, measured); the others keep the
-// compiler's allocation (capping them spills more than it gains).
+// Register budget: three waves per SIMD for the FP64 hydro kernel and the
+// FP64 MHD halves, three fields each (their few spills cost less than the
+// lost occupancy, measured for hydro); the others keep the compiler's
+// allocation (capping them spills more than it gains).
 template <typename R, int KIND>
 constexpr int xr_waves_per_eu() {
-  return (sizeof(R) == 8 && KIND == XR_HYDRO) ? 3 : (KIND == XR_ADJ ? 2 : 1);
+  return (sizeof(R) == 8 && KIND != XR_ADJ) ? 3 : (KIND == XR_ADJ ? 2 : 1);
 }
 
 template <typename R, int N, bool MHD>

@@ -41,6 +41,9 @@ int fail(int code, const std::string& msg) {
   extern const tg::EnsTable<float> tg_ens_32_##n;
 TG_GRID_SIZES(TG_DECL_TABLES)
 #undef TG_DECL_TABLES
+// Mixed-size ensemble launches (tg_ens_mixed.hip).
+extern const tg::EnsMixTable<double> tg_ens_mix_64;
+extern const tg::EnsMixTable<float> tg_ens_mix_32;
 // Every other size: the direct-sum path of tg_generic.hip.
 extern const tg::OpsTable<double> tg_ops_64_generic;
 extern const tg::OpsTable<float> tg_ops_32_generic;
@@ -90,6 +93,13 @@ const EnsTable<float>* ens_table<float>(int n) {
 #undef TG_CASE
   return nullptr;
 }
+
+template <typename R>
+const EnsMixTable<R>* mix_table();
+template <>
+const EnsMixTable<double>* mix_table<double>() { return &tg_ens_mix_64; }
+template <>
+const EnsMixTable<float>* mix_table<float>() { return &tg_ens_mix_32; }
 
 // Call c(table, (R*)nullptr) with the plan's entry-point table.
 template <typename Call>
@@ -401,6 +411,12 @@ struct tg_batch {
   std::vector<int> order;            // descriptor slot -> instance (grouped by size)
   std::vector<int> cls_n, cls_first, cls_count;  // launch classes (one grid size each) over descriptor slots
   std::vector<size_t> cls_prefix_off;            // offset of each class's EP_COUNT x (count+1) table
+  // mixed group: the contiguous descriptor slots of the kMixSizes instances
+  // (largest size first), with its own prefix tables; `mixed` selects one
+  // launch per pass for the group instead of one per class
+  int mix_first = 0, mix_count = 0;
+  size_t mix_prefix_off = 0;
+  bool mixed = false;
   std::vector<int> prefix;
   std::vector<void*> tw_by_size;     // twiddles per entry of kEnsSizes
   void* d_inst = nullptr;            // EnsInst<R>[ninst]
@@ -434,25 +450,52 @@ int batch_class_step(tg_batch* b, int c, double dt, hipStream_t s) {
   return TG_OK;
 }
 
-// One RK4 step of every instance.  Size classes are independent: with
-// `branches` (the graph capture) class c >= 1 runs on its own stream forked
+// The four stages of one RK4 step of the mixed group on stream s.
+template <typename R>
+int batch_mix_step(tg_batch* b, double dt, hipStream_t s) {
+  const auto* inst = static_cast<const EnsInst<R>*>(b->d_inst) + b->mix_first;
+  const size_t po = b->mix_prefix_off;
+  for (int st = 0; st < 4; ++st)
+    TG_RET(mix_table<R>()->stage(inst, b->d_prefix + po, b->prefix.data() + po, b->mix_count, st, dt, s));
+  return TG_OK;
+}
+
+// Launch units of one step: the mixed group (when selected) and every class
+// outside it, or every class.  Unit -1 is the mixed group.
+std::vector<int> batch_units(const tg_batch* b) {
+  std::vector<int> u;
+  const bool mix = b->mixed && b->mix_count > 0;
+  if (mix) u.push_back(-1);
+  for (int c = 0; c < int(b->cls_n.size()); ++c)
+    if (!mix || !is_mix_size(b->cls_n[c])) u.push_back(c);
+  return u;
+}
+
+template <typename R>
+int batch_unit_step(tg_batch* b, int unit, double dt, hipStream_t s) {
+  return unit < 0 ? batch_mix_step<R>(b, dt, s) : batch_class_step<R>(b, unit, dt, s);
+}
+
+// One RK4 step of every instance.  Launch units are independent: with
+// `branches` (the graph capture) unit u >= 1 runs on its own stream forked
 // from and joined back to s, so the captured graph holds one parallel
-// branch per class; without, the classes run one after the other on s.
+// branch per unit; without, the units run one after the other on s.
 template <typename R>
 int batch_launch_step(tg_batch* b, double dt, hipStream_t s, bool branches) {
-  const int ncls = int(b->cls_n.size());
-  if (!branches || ncls == 1) {
-    for (int c = 0; c < ncls; ++c) TG_RET(batch_class_step<R>(b, c, dt, s));
+  const std::vector<int> units = batch_units(b);
+  const int nu = int(units.size());
+  if (!branches || nu == 1) {
+    for (int u : units) TG_RET(batch_unit_step<R>(b, u, dt, s));
     return TG_OK;
   }
   TG_HIP(hipEventRecord(b->fork[0], s));
-  for (int c = 1; c < ncls; ++c) {
-    TG_HIP(hipStreamWaitEvent(b->branch[c - 1], b->fork[0], 0));
-    TG_RET(batch_class_step<R>(b, c, dt, b->branch[c - 1]));
-    TG_HIP(hipEventRecord(b->fork[c], b->branch[c - 1]));
+  for (int k = 1; k < nu; ++k) {
+    TG_HIP(hipStreamWaitEvent(b->branch[k - 1], b->fork[0], 0));
+    TG_RET(batch_unit_step<R>(b, units[k], dt, b->branch[k - 1]));
+    TG_HIP(hipEventRecord(b->fork[k], b->branch[k - 1]));
   }
-  TG_RET(batch_class_step<R>(b, 0, dt, s));
-  for (int c = 1; c < ncls; ++c) TG_HIP(hipStreamWaitEvent(s, b->fork[c], 0));
+  TG_RET(batch_unit_step<R>(b, units[0], dt, s));
+  for (int k = 1; k < nu; ++k) TG_HIP(hipStreamWaitEvent(s, b->fork[k], 0));
   return TG_OK;
 }
 
@@ -524,7 +567,7 @@ int batch_steps(tg_batch* b, int nsteps, double dt, int use_graph, hipStream_t s
       b->exec = nullptr;
     }
     if (!b->cap) TG_HIP(hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking));
-    const size_t ncls = b->cls_n.size();
+    const size_t ncls = batch_units(b).size();
     while (b->branch.size() + 1 < ncls) {
       hipStream_t t = nullptr;
       TG_HIP(hipStreamCreateWithFlags(&t, hipStreamNonBlocking));
@@ -592,7 +635,9 @@ int tg_batch_create(int ninst, const int* n, const double* nu, int precision_bit
     b->ws_off[i + 1] = b->ws_off[i] + 3 * align256(vec) + align256(ae * b->cb) + align256(be * b->cb);
   }
 
-  for (int k = 0; k < kNumEnsSizes; ++k) {
+  // descriptor slots grouped by size, largest first (the mixable sizes are
+  // then contiguous, and the largest blocks of a mixed launch go first)
+  for (int k = kNumEnsSizes - 1; k >= 0; --k) {
     int cnt = 0;
     for (int i = 0; i < ninst; ++i)
       if (n[i] == kEnsSizes[k]) {
@@ -631,6 +676,30 @@ int tg_batch_create(int ninst, const int* n, const double* nu, int precision_bit
       const int per = precision_bits == 64 ? ens_table<double>(b->cls_n[c])->blocks[pass]
                                            : ens_table<float>(b->cls_n[c])->blocks[pass];
       for (int j = 0; j <= cnt; ++j) b->prefix.push_back(j * per);
+    }
+  }
+  // the mixed group's prefix tables: every kMixSizes instance, in slot order
+  {
+    int first = -1, cnt = 0;
+    for (int slot = 0; slot < ninst; ++slot)
+      if (is_mix_size(n[b->order[slot]])) {
+        if (first < 0) first = slot;
+        ++cnt;
+      }
+    if (cnt > 0) {
+      b->mix_first = first;
+      b->mix_count = cnt;
+      b->mix_prefix_off = b->prefix.size();
+      for (int pass = 0; pass < EP_COUNT; ++pass) {
+        int acc = 0;
+        b->prefix.push_back(0);
+        for (int j = 0; j < cnt; ++j) {
+          const int N = n[b->order[first + j]];
+          acc += precision_bits == 64 ? ens_table<double>(N)->blocks[pass] : ens_table<float>(N)->blocks[pass];
+          b->prefix.push_back(acc);
+        }
+      }
+      b->mixed = true;
     }
   }
   auto cleanup = [&](hipError_t e, const char* what) {
@@ -690,6 +759,17 @@ int tg_batch_query(const tg_batch* b, long long* state_offsets, long long* ws_by
   if (state_offsets)
     for (int i = 0; i <= b->ninst; ++i) state_offsets[i] = b->state_off[i];
   if (ws_bytes) *ws_bytes = (long long)b->ws_off[b->ninst];
+  return TG_OK;
+}
+
+int tg_batch_set_launch(tg_batch* b, int mixed) {
+  if (!b) return fail(TG_ERR_ARG, "batch is NULL");
+  const bool m = mixed != 0;
+  if (m != b->mixed && b->exec) {
+    TG_HIP(hipGraphExecDestroy(b->exec));
+    b->exec = nullptr;
+  }
+  b->mixed = m;
   return TG_OK;
 }
 

@@ -12,6 +12,7 @@ Leray-projected, scaled to rms 0.05 per component).
 """
 import ctypes
 import math
+import time
 
 import numpy as np
 import torch
@@ -78,6 +79,8 @@ class Ensemble:
             _lib._check(lib.tg_batch_create(n, (ctypes.c_int * n)(*self.sizes), (ctypes.c_double * n)(*self.nus),
                                             64 if dtype == torch.float64 else 32, self.L, ctypes.byref(h)))
         self._h = h
+        # the library's default layout: mixed launches when any size 32-96 is present
+        self.mixed = any(N in (32, 48, 64, 96) for N in self.sizes)
         offs = (ctypes.c_longlong * (n + 1))()
         wsb = ctypes.c_longlong()
         _lib._check(lib.tg_batch_query(h, offs, ctypes.byref(wsb)))
@@ -110,6 +113,32 @@ class Ensemble:
 
     def velocity(self, i):
         return self.plan(i).inverse(self.views[i])
+
+    def set_launch(self, mixed):
+        """Launch layout of a step: mixed=True runs one launch per pass for
+        every instance of the sizes 32, 48, 64 and 96 (their batched bodies
+        share block shapes; other sizes keep their own launches); False runs
+        one launch per pass and size class, the classes as parallel graph
+        branches."""
+        _lib._check(self._lib.tg_batch_set_launch(self._h, int(bool(mixed))))
+        self.mixed = bool(mixed)
+
+    def tune_launch(self, dt=5e-3, steps=10):
+        """Time both launch layouts (graph replays, which advance the current
+        state: call before setting the initial fields) and keep the faster.
+        Returns {"mixed": s, "classes": s, "chosen": "mixed" | "classes"}."""
+        out = {}
+        for name, mixed in (("classes", False), ("mixed", True)):
+            self.set_launch(mixed)
+            self.step(1, dt, graph=True)  # capture
+            torch.cuda.synchronize(self.device)
+            t0 = time.perf_counter()
+            self.step(steps, dt, graph=True)
+            torch.cuda.synchronize(self.device)
+            out[name] = time.perf_counter() - t0
+        out["chosen"] = "mixed" if out["mixed"] <= out["classes"] else "classes"
+        self.set_launch(out["chosen"] == "mixed")
+        return out
 
     def step(self, nsteps=1, dt=5e-3, graph=True):
         with torch.cuda.device(self.device):

@@ -1,7 +1,9 @@
 """Batched ensemble: every instance of a mixed-size batch matches the FP64
 oracle (advective form where dealiasing is exact, rotational form for
 N = 48, 96), the batched kernels match the single-instance plans to
-round-off, and graph replays match eager execution bitwise on every replay."""
+round-off, graph replays match eager execution bitwise on every replay, and
+both launch layouts (one launch per pass for the sizes 32-96, or one per
+size class) agree."""
 import numpy as np
 import pytest
 import torch
@@ -14,19 +16,21 @@ L = 2 * np.pi
 SIZES = [16, 32, 48, 64, 96, 128]
 
 
-def _ens(dtype, sizes=SIZES, seed=5):
+def _ens(dtype, sizes=SIZES, seed=5, mixed=True):
     from jax_tg_project_amd import ensemble
     rng = np.random.default_rng(seed)
     nus = [float(np.exp(rng.uniform(np.log(1 / 1600), np.log(1 / 100)))) for _ in sizes]
     ens = ensemble.Ensemble(sizes, nus, dtype=dtype)
+    ens.set_launch(mixed)
     u0 = [ensemble.perturbed_taylor_green(N, 100 + i) for i, N in enumerate(sizes)]
     for i, u in enumerate(u0):
         ens.set_velocity(i, u)
     return ens, u0, nus
 
 
-def test_ensemble_fp64_matches_oracle_per_instance():
-    ens, u0, nus = _ens(torch.float64)
+@pytest.mark.parametrize("mixed", [True, False], ids=["mixed", "classes"])
+def test_ensemble_fp64_matches_oracle_per_instance(mixed):
+    ens, u0, nus = _ens(torch.float64, mixed=mixed)
     dt, steps = 5e-3, 3
     ens.step(steps, dt, graph=False)
     for i, N in enumerate(SIZES):
@@ -55,9 +59,10 @@ def test_ensemble_matches_single_instance_plans(dtype, tol):
         assert err < tol, (N, err)
 
 
-def test_graph_replays_match_eager_every_replay():
-    g, _, _ = _ens(torch.float32, seed=9)
-    e, _, _ = _ens(torch.float32, seed=9)
+@pytest.mark.parametrize("mixed", [True, False], ids=["mixed", "classes"])
+def test_graph_replays_match_eager_every_replay(mixed):
+    g, _, _ = _ens(torch.float32, seed=9, mixed=mixed)
+    e, _, _ = _ens(torch.float32, seed=9, mixed=mixed)
     for rep in range(4):
         g.step(1, 5e-3, graph=True)
         torch.cuda.synchronize()
@@ -73,6 +78,36 @@ def test_graph_replays_match_eager_every_replay():
     e.step(1, 2.5e-3, graph=False)
     torch.cuda.synchronize()
     assert torch.equal(g.state, e.state)
+
+
+@pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-14), (torch.float32, 1e-6)], ids=["fp64", "fp32"])
+def test_launch_layouts_agree_on_the_seeded_mix(dtype, tol):
+    """A 24-instance draw of the seeded workload (several instances per size,
+    so the mixed launch's block table spans many descriptors) stepped with
+    both layouts, graph and eager; the layout switch re-captures the graph."""
+    from jax_tg_project_amd import ensemble
+    cfg = ensemble.ensemble_config(24, seed=4)
+    sizes = [c[0] for c in cfg]
+    runs = []
+    for mixed in (True, False):
+        ens = ensemble.build_ensemble(cfg, dtype=dtype)
+        ens.set_launch(mixed)
+        ens.step(3, 5e-3, graph=True)
+        ens.step(2, 5e-3, graph=False)
+        runs.append(ens)
+    torch.cuda.synchronize()
+    for i, N in enumerate(sizes):
+        a, b = runs[0].views[i], runs[1].views[i]
+        err = (torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b)).item()
+        assert err <= tol, (i, N, err)
+    g = runs[0]
+    g.set_launch(False)
+    g.step(1, 5e-3, graph=True)
+    runs[1].step(1, 5e-3, graph=True)
+    torch.cuda.synchronize()
+    for i in range(len(sizes)):
+        err = (torch.linalg.vector_norm(g.views[i] - runs[1].views[i]) / torch.linalg.vector_norm(runs[1].views[i])).item()
+        assert err <= tol, (i, err)
 
 
 def test_ensemble_fp32_energy_vs_fp64():
