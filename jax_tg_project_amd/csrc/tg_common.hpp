@@ -66,6 +66,10 @@ struct tg_plan {
   // adjoint: private stream (default priority) on which the reverse sweep
   // recomputes the next segment while the current one is reversed
   hipStream_t aux = nullptr;
-  hipEvent_t adjev[3] = {nullptr, nullptr, nullptr};
+  // one event per use: adjev[0] orders the sweep's start; segment c's
+  // recompute-done and reverse-done events are adjev[1 + 2c], adjev[2 + 2c]
+  // (an event is recorded once per sweep, so every wait names the record it
+  // was enqueued after, however far the host runs ahead of the GPU)
+  std::vector<hipEvent_t> adjev;
   const void* ops = nullptr;  // OpsTable<R> of (prec, n)
 };

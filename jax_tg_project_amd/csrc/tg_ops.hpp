@@ -468,24 +468,28 @@ struct Ops {
   static int steps_keeping_stages(const tg_plan* p, cpx<R>* cur, cpx<R>* st, size_t vstride, int len, double dt,
                                   double nu, void* ws, hipStream_t s, cpx<R>* slots = nullptr,
                                   size_t sstride = 0) {
-    Ptrs<R> w = carve<R>(p, ws);
     const size_t bytes = size_t(3) * p->K * p->K * kzp<R, N>() * sizeof(cpx<R>);
     if (len > 0) TG_HIP(hipMemcpyAsync(st, cur, bytes, hipMemcpyDeviceToDevice, s));
-    for (int j = 0; j < len; ++j) {
-      // step j starts from st[4j]; its last stage writes the next step's
-      // input slot directly (cur only after the last step)
-      cpx<R>* S = st + size_t(4 * j) * vstride;
-      for (int stg = 0; stg < 4; ++stg) {
-        StageArgs<R> a{};
-        a.u0 = S;
-        a.us = S + size_t(stg) * vstride;
-        a.unext = stg < 3 ? S + size_t(stg + 1) * vstride : (j + 1 < len ? S + size_t(4) * vstride : cur);
-        a.uacc = w.Acc;
-        a.dt = R(dt); a.nu = R(nu); a.ks = R(p->ks); a.stage = stg;
-        a.cpz = kzp<R, N>();
-        cpx<R>* slot = slots ? slots + size_t(4 * j + stg) * sstride : nullptr;
-        TG_RET(rhs_passes<R, N>(p, w, a.us, nullptr, FM_HYDRO_STAGE, a, R(0), s, slot));
-      }
+    for (int j = 0; j < len; ++j) TG_RET(keep_step(p, cur, st, vstride, j, len, dt, nu, ws, s, slots, sstride));
+    return TG_OK;
+  }
+  // Step j of steps_keeping_stages (after its copy st[0] <- cur): it starts
+  // from st[4j]; its last stage writes the next step's input slot directly
+  // (cur only after the last step).
+  static int keep_step(const tg_plan* p, cpx<R>* cur, cpx<R>* st, size_t vstride, int j, int len, double dt,
+                       double nu, void* ws, hipStream_t s, cpx<R>* slots, size_t sstride) {
+    Ptrs<R> w = carve<R>(p, ws);
+    cpx<R>* S = st + size_t(4 * j) * vstride;
+    for (int stg = 0; stg < 4; ++stg) {
+      StageArgs<R> a{};
+      a.u0 = S;
+      a.us = S + size_t(stg) * vstride;
+      a.unext = stg < 3 ? S + size_t(stg + 1) * vstride : (j + 1 < len ? S + size_t(4) * vstride : cur);
+      a.uacc = w.Acc;
+      a.dt = R(dt); a.nu = R(nu); a.ks = R(p->ks); a.stage = stg;
+      a.cpz = kzp<R, N>();
+      cpx<R>* slot = slots ? slots + size_t(4 * j + stg) * sstride : nullptr;
+      TG_RET(rhs_passes<R, N>(p, w, a.us, nullptr, FM_HYDRO_STAGE, a, R(0), s, slot));
     }
     return TG_OK;
   }
@@ -542,16 +546,19 @@ struct Ops {
     // buffer while segment c is reversed on `s`; events order the buffer
     // reuse both ways.
     hipStream_t s2 = nullptr;
-    hipEvent_t ev_in = nullptr, ev_rec = nullptr, ev_rev = nullptr;
+    tg_plan* mp = const_cast<tg_plan*>(p);
+    auto ev_rec = [&](int c) { return mp->adjev[1 + 2 * size_t(c)]; };  // recompute of segment c done
+    auto ev_rev = [&](int c) { return mp->adjev[2 + 2 * size_t(c)]; };  // reverse of segment c done
     if (L.nck > 1) {
-      tg_plan* mp = const_cast<tg_plan*>(p);
       if (!mp->aux) TG_HIP(hipStreamCreateWithFlags(&mp->aux, hipStreamNonBlocking));
-      for (hipEvent_t& e : mp->adjev)
-        if (!e) TG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      while (mp->adjev.size() < 1 + 2 * size_t(L.nck)) {
+        hipEvent_t e = nullptr;
+        TG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        mp->adjev.push_back(e);
+      }
       s2 = mp->aux;
-      ev_in = mp->adjev[0]; ev_rec = mp->adjev[1]; ev_rev = mp->adjev[2];
-      TG_HIP(hipEventRecord(ev_in, s));  // checkpoints written, nupart cleared
-      TG_HIP(hipStreamWaitEvent(s2, ev_in, 0));
+      TG_HIP(hipEventRecord(mp->adjev[0], s));  // checkpoints written, nupart cleared
+      TG_HIP(hipStreamWaitEvent(s2, mp->adjev[0], 0));
     }
     for (int c = L.nck - 1; c >= 0; --c) {
       const int len = std::min(seg, nsteps - c * seg);
@@ -559,18 +566,23 @@ struct Ops {
       cpx<R>* stc = even ? L.st : L.st2;
       cpx<R>* slc = even ? L.slots : L.slots2;
       // segment c recomputed (the last two were kept by the forward sweep)
-      if (c < L.nck - 2) TG_HIP(hipStreamWaitEvent(s, ev_rec, 0));
-      if (c > 0 && c - 1 != L.nck - 2) {
-        // recompute segment c - 1 into the buffer segment c + 1 used
-        const bool evn = (L.nck - c) % 2 == 0;
-        cpx<R>* stn = evn ? L.st : L.st2;
-        cpx<R>* sln = evn ? L.slots : L.slots2;
-        const int lenn = std::min(seg, nsteps - (c - 1) * seg);
+      if (c < L.nck - 2) TG_HIP(hipStreamWaitEvent(s, ev_rec(c), 0));
+      // recompute segment c - 1 into the buffer segment c + 1 used; its steps
+      // are enqueued one per reversed step of segment c, so the host feeds
+      // both streams together instead of queueing the whole recompute first
+      const bool rec = c > 0 && c - 1 != L.nck - 2;
+      const bool evn = (L.nck - c) % 2 == 0;
+      cpx<R>* stn = evn ? L.st : L.st2;
+      cpx<R>* sln = evn ? L.slots : L.slots2;
+      const int lenn = rec ? std::min(seg, nsteps - (c - 1) * seg) : 0;
+      int jr = 0;
+      if (rec) {
         TG_HIP(hipMemcpyAsync(L.cur2, L.ck + (c - 1) * L.vstride, bytes, hipMemcpyDeviceToDevice, s2));
-        TG_RET(steps_keeping_stages(p, L.cur2, stn, L.vstride, lenn, dt, nu, L.ws2, s2, sln, L.sstride));
-        TG_HIP(hipEventRecord(ev_rec, s2));
+        TG_HIP(hipMemcpyAsync(stn, L.cur2, bytes, hipMemcpyDeviceToDevice, s2));
       }
       for (int j = len - 1; j >= 0; --j) {
+        for (; jr < lenn && jr * len <= (len - 1 - j) * lenn; ++jr)
+          TG_RET(keep_step(p, L.cur2, stn, L.vstride, jr, lenn, dt, nu, L.ws2, s2, sln, L.sstride));
         cpx<R>* S = stc + size_t(4 * j) * L.vstride;
         for (int sidx = 3; sidx >= 0; --sidx) {
           StageArgs<R> a{};
@@ -586,10 +598,13 @@ struct Ops {
         }
         std::swap(lamp, lacc);
       }
+      for (; jr < lenn; ++jr)
+        TG_RET(keep_step(p, L.cur2, stn, L.vstride, jr, lenn, dt, nu, L.ws2, s2, sln, L.sstride));
+      if (rec) TG_HIP(hipEventRecord(ev_rec(c - 1), s2));
       if (c > 1) {
         // the recompute of segment c - 2 reuses this buffer: after this reverse
-        TG_HIP(hipEventRecord(ev_rev, s));
-        TG_HIP(hipStreamWaitEvent(s2, ev_rev, 0));
+        TG_HIP(hipEventRecord(ev_rev(c), s));
+        TG_HIP(hipStreamWaitEvent(s2, ev_rev(c), 0));
       }
     }
     hipLaunchKernelGGL((k_adj_final<R, N>), dim3(cdiv(int(p->compact), 256)), dim3(256), 0, s, lamp, grad, KZP);

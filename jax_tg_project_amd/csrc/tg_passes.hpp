@@ -78,11 +78,19 @@ constexpr int xy_cap_l(int l, int cap) {
 }
 template <typename R, int N>
 constexpr bool xy_wide() { return lds_fits<R>(long(N) * 33) && 32 * N / base_e<N>() <= 512; }
+// At N = 128 variant 2 is instead one line of columns x 4 points: a 128^3
+// grid has only K * KZ / 8 = 457 FP64 column lines, so 8 or 16 points per
+// thread leave about one wavefront per SIMD; 4 points double the wavefronts.
+template <int N>
+constexpr bool xy_narrow() { return N <= 128 && N % 3 != 0; }
 template <typename R, int N, int V>
 struct CfgXY {
-  static constexpr int E = V == 1 ? alt_e<N>() : (V == 2 && !xy_wide<R, N>()) ? 2 * base_e<N>() : base_e<N>();
+  static constexpr int E = V == 1 ? alt_e<N>()
+                         : V == 2 ? (xy_narrow<N>() ? 4 : !xy_wide<R, N>() ? 2 * base_e<N>() : base_e<N>())
+                         : base_e<N>();
   static constexpr int L = V == 1 ? xy_cap_l<N, E>(sizeof(R) == 4 ? 16 : 8, 512)
-                         : V == 2 ? (xy_wide<R, N>() ? 32 : 16) : xy_cap_l<N, E>(16, 256);
+                         : V == 2 ? (xy_narrow<N>() ? (sizeof(R) == 4 ? 16 : 8) : xy_wide<R, N>() ? 32 : 16)
+                         : xy_cap_l<N, E>(16, 256);
   static constexpr int NT = L * N / E;
   static_assert(NT <= 1024 && lds_fits<R>(long(N) * (L + 1)), "x/y tile exceeds a workgroup");
 };
@@ -1558,7 +1566,9 @@ __global__ __launch_bounds__((CfgXF<R, N, V>::NT)) void k_fwd_x(const cpx<R>* __
 // blocks share a CU and overlap their load, transform and update phases.
 template <typename R, int N>
 struct CfgXR {
-  static constexpr int E = 8;  // power-of-two grids of N >= 128 only (use_xr)
+  // power-of-two grids of N >= 128 only (use_xr); 4 points per thread at
+  // N = 128 (twice the wavefronts of 8 on the small grid), else 8
+  static constexpr int E = N <= 128 ? 4 : 8;
   // one 128-byte line of kz per row, halved in FP32 above 512 threads
   static constexpr int L = (sizeof(R) == 8 || 16 * N / E <= 512) ? 128 / int(2 * sizeof(R)) : 8;
   static constexpr int NT = L * N / E;
