@@ -156,6 +156,17 @@ def cpu_baseline(N, dt, nu, gpu_one_step=None):
     return out, parity
 
 
+def leg_roofline(passes, profile):
+    """Roofline object of a leg from its HIP-event pass timings: the pass
+    with the largest time per launch, its algorithmic GB/s over 8 TB/s.
+    Measured traffic of the leg's kernels is in the committed summary
+    `profile` (rocprofv3 counters), not re-measured here."""
+    dom = max(passes, key=lambda k: passes[k]["ms_avg"])
+    a = passes[dom]["GBps"]
+    return {"bound": "hbm", "kernel": dom, "achieved": a, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": a / HBM_PEAK_GBS, "traffic": None, "profile": profile}
+
+
 def run_mhd(N, steps, dt, nu, eta, dtype, lorentz_sign=1.0):
     """MHD TGV: u = TG, b = (cos x sin y sin z, sin x cos y sin z,
     -2 sin x sin y cos z) (= -curl u_TG), nu = eta, `steps` RK4 steps on the
@@ -195,6 +206,7 @@ def run_mhd(N, steps, dt, nu, eta, dtype, lorentz_sign=1.0):
     step_b = roofline.step_bytes(N, 8 if dtype == torch.float64 else 4, mhd=True)
     return {"steps_per_s": steps / wall, "wall_s": wall, "tile_variants": tuned, "passes": passes,
             "hbm_GBps_algorithmic_step": step_b * steps / wall / 1e9,
+            "roofline": leg_roofline(passes, "profiles/mhd_128_kernel_stats_and_traffic.md"),
             "kinetic_energy": d["kinetic_energy"], "magnetic_energy": d["magnetic_energy"],
             "cross_helicity": d["cross_helicity"], "div_u_max": d["div_u_max"], "div_b_max": d["div_b_max"]}
 
@@ -226,7 +238,11 @@ def run_ensemble(ninst, steps, dt, seed=0):
     same = bool(torch.equal(ens.state, e_eager.state))
     nbytes = sum(roofline.step_bytes(N, 4) for N, _, _ in cfg) * steps
     counts = {n: sum(1 for c in cfg if c[0] == n) for n in ensemble.ENSEMBLE_SIZES}
-    return {"value": ninst / wall, "unit": "solves/s", "instances": ninst, "steps": steps, "dt": dt,
+    rate = nbytes / wall / 1e9
+    return {"roofline": {"bound": "hbm", "kernel": "whole graph step (20 mixed launches)", "achieved": rate,
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": rate / HBM_PEAK_GBS, "traffic": None,
+                         "profile": "profiles/ensemble_kernel_stats_and_traffic.md"},
+            "value": ninst / wall, "unit": "solves/s", "instances": ninst, "steps": steps, "dt": dt,
             "seed": seed, "sizes": counts, "launch_layout": layout, "wall_s": wall, "wall_eager_s": wall_eager,
             "graph_equals_eager": same, "hbm_GBps_algorithmic": nbytes / wall / 1e9}
 
@@ -398,8 +414,12 @@ def main():
         if "energy_fp64_rel_err_vs_oracle" in res:
             res["fp32_energy_rel_err_vs_oracle"] = abs(r32["diag"]["kinetic_energy"] - hist["energy"][-1]) / \
                 hist["energy"][-1]
-        res["passes_fp32"] = {k: {"ms_avg": v[0] / max(1, v[1]), "launches": v[1]}
+        pb32 = roofline.pass_bytes(N, 4)
+        res["passes_fp32"] = {k: {"ms_avg": v[0] / max(1, v[1]), "launches": v[1],
+                                  "GBps": ((pb32[k] if k != "fwd_x" else sum(pb32["fwd_x"]) / 4.0)
+                                           / (v[0] / max(1, v[1]) * 1e-3) / 1e9)}
                               for k, v in r32["passes"].items()}
+        res["roofline_fp32"] = leg_roofline(res["passes_fp32"], "profiles/hydro_256_kernel_stats.md")
         res["tile_variants_fp32"] = r32["variants"]
         del r32
         torch.cuda.empty_cache()
@@ -423,6 +443,7 @@ def main():
                           "fp32_over_fp64": m32["steps_per_s"] / m64["steps_per_s"]}
     if not args.no_adjoint:
         res["adjoint"] = [run_adjoint(128, 200, 5e-3, 1.0 / 400, dt_) for dt_ in (torch.float64, torch.float32)]
+        res["adjoint_profile"] = "profiles/adjoint_128_kernel_stats_and_traffic.md"
         # the headline grid: 100 steps of the 256^3 TGV (keeping every stage would need ~0.27 TB in FP64)
         res["adjoint"] += [run_adjoint(256, 100, dt, nu, dt_, all_stages=False) for dt_ in (torch.float64, torch.float32)]
         torch.cuda.empty_cache()
