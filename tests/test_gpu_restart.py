@@ -97,14 +97,19 @@ def test_native_cli_matches_the_solver_and_restarts(tmp_path):
     assert rc == 1 and "dt=" in err
 
 
-def test_native_cli_fp32_default_and_unsupported_size():
+def test_native_cli_fp32_default_generic_and_unsupported_size():
     rc, out, err = _cli(32, 0.005, 1000, 0.02, 0.01)
     assert rc == 0, err
     rows = [LINE.match(l).groups() for l in out.splitlines()[1:]]
     assert [int(r[0]) for r in rows] == [2, 4]
     assert abs(float(rows[0][3]) - 0.125) < 1e-3 and abs(float(rows[0][2]) - 0.375) < 1e-2
-    rc, _, err = _cli(20, 0.005, 1000, 0.02, 0.01)  # 5 divides 20: no compiled plan
-    assert rc == 2 and "N=20" in err
+    # 5 divides 20: no compiled kernels, the direct-sum path runs it
+    rc, out, err = _cli(20, 0.005, 1000, 0.02, 0.01)
+    assert rc == 0, err
+    rows = [LINE.match(l).groups() for l in out.splitlines()[1:]]
+    assert [int(r[0]) for r in rows] == [2, 4] and abs(float(rows[0][3]) - 0.125) < 1e-3
+    rc, _, err = _cli(4096, 0.005, 1000, 0.02, 0.01)  # above the largest grid a plan accepts
+    assert rc == 2 and "N=4096" in err
 
 
 def test_python_package_clis(tmp_path, monkeypatch):
