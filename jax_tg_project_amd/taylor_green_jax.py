@@ -15,6 +15,14 @@ reference's run loop: the TG initial condition and all RK4 updates lie
 inside the mask).  An input whose part outside the mask exceeds round-off
 (relative amplitude BAND_TOL) raises ValueError instead of silently
 differing from the reference, which would transform it unmasked.
+
+The compiled kernels form the nonlinear term as u x omega.  After the 2/3
+truncation and the projection this equals the reference's u . grad u
+whenever 3 (N // 3) < N.  Where 3 divides a compiled N (12, 24, 48, 96, 192,
+384) the inclusive mask keeps the shell |k_i| = N/3 that the product aliases
+into, and there the two forms differ, by about 1e-5 relative on a random
+band-limited field.  Sizes without compiled kernels run the direct-sum path,
+which forms the reference's advective term exactly.
 """
 import math
 import time
@@ -290,6 +298,8 @@ def run_simulation(N=16, dt=0.005, nu=None, reynolds=None, total_time=0.5, save_
     FP32 variant; ``make_plots`` switches the reference's PNG side effect;
     with ``check_finite`` a save point whose state holds NaN or infinite
     values raises FloatingPointError (the reference would carry on with NaN).
+    Where 3 divides a compiled N the nonlinear term's aliased shell differs
+    from the reference's advective form (module docstring).
     """
     from .taylor_green_cuda import check_finite as _finite
     if N < 4 or total_time <= 0.0 or save_every_time <= 0.0:
