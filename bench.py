@@ -258,6 +258,7 @@ def run_adjoint(N, steps, dt, nu, dtype, all_stages=True):
     from jax_tg_project_amd import autotune
     plan = Plan(N, dtype=dtype)
     tuned = autotune.tune(plan, use_cache=False)[0]  # forward and reverse sweeps run the tuned tiles
+    rtuned = autotune.tune_recompute(plan, use_cache=False)[0]  # the recompute beside the reverse sweep
     x0 = plan.forward(torch.tensor(ensemble.perturbed_taylor_green(N, 2024), dtype=dtype, device=plan.device))
     seg = sqrt_segment(steps)
     y = x0.clone()
@@ -270,7 +271,7 @@ def run_adjoint(N, steps, dt, nu, dtype, all_stages=True):
     torch.cuda.synchronize()
     fwd = time.perf_counter() - t0
     out = {"N": N, "steps": steps, "dtype": "f64" if dtype == torch.float64 else "f32", "tile_variants": tuned,
-           "forward_s": fwd}
+           "recompute_tile_variants": rtuned, "forward_s": fwd}
     for name, sg in (("sqrt_n", seg), ("all_stages_kept", steps))[:2 if all_stages else 1]:
         aws = plan.adjoint_workspace(steps, sg)
         torch.cuda.synchronize()

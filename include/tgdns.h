@@ -169,6 +169,16 @@ int tg_timing_enable(tg_plan* plan, int on);
  * the same butterflies; the autotuner (jax_tg_project_amd/autotune.py)
  * times them and stores the fastest per (device, N, precision, equations). */
 int tg_plan_variants(tg_plan* plan, const int* set, int* get, int* count);
+
+/* Tile variants of the adjoint recompute (no counterpart in the reference:
+ * tuning).  The reverse sweep of tg_adjoint_backward recomputes each earlier
+ * checkpoint segment on a second stream while it reverses the current one;
+ * those RHS passes may use their own variants, picked for running beside the
+ * reverse sweep's kernels rather than alone.  set[0] = -1 returns them to
+ * the plan's own (tg_plan_variants); `get` receives the effective set.  The
+ * recomputed stages then differ from a recompute with the plan's variants
+ * by round-off only. */
+int tg_plan_recompute_variants(tg_plan* plan, const int* set, int* get);
 int tg_timing_read(tg_plan* plan, double* ms, long long* count);
 
 /* ---- Batched ensemble (no counterpart in the reference, which advances one

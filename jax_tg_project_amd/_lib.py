@@ -51,6 +51,8 @@ _SIGS = {
     "tg_diagnostics_noalloc": ([_VP, _VP, _VP, ctypes.c_int, _VP, _VP], ctypes.c_int),
     "tg_plan_variants": ([_VP, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                           ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+    "tg_plan_recompute_variants": ([_VP, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)],
+                                   ctypes.c_int),
     "tg_timing_enable": ([_VP, ctypes.c_int], ctypes.c_int),
     "tg_timing_read": ([_VP, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_longlong)],
                        ctypes.c_int),
@@ -351,6 +353,19 @@ class Plan:
     def set_variants(self, variants):
         arr = (ctypes.c_int * len(PASS_NAMES))(*[int(v) for v in variants])
         _check(self._lib.tg_plan_variants(self._h, arr, None, None))
+
+    def get_recompute_variants(self):
+        """Tile variants of the adjoint recompute stream (the plan's own
+        unless set_recompute_variants chose others)."""
+        out = (ctypes.c_int * len(PASS_NAMES))()
+        _check(self._lib.tg_plan_recompute_variants(self._h, None, out))
+        return [int(v) for v in out]
+
+    def set_recompute_variants(self, variants):
+        """variants: one per pass, or None for the plan's own."""
+        vals = [-1] * len(PASS_NAMES) if variants is None else [int(v) for v in variants]
+        arr = (ctypes.c_int * len(PASS_NAMES))(*vals)
+        _check(self._lib.tg_plan_recompute_variants(self._h, arr, None))
 
     def timing(self, on):
         """Enable/disable the per-pass HIP-event timing hook."""

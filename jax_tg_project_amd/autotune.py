@@ -11,6 +11,7 @@ equations), so later processes skip the measurement.
 import json
 import math
 import os
+import time
 
 import torch
 
@@ -101,3 +102,62 @@ def tune(plan, steps=3, use_cache=True, verbose=False, rounds=3):
         cache[key] = {"variants": best, "ms": ms}
         _store(cache)
     return best, ms
+
+
+def tune_recompute(plan, nsteps=64, seg=8, use_cache=True, verbose=False, rounds=2):
+    """Pick the tile variants of the adjoint recompute, which runs on its own
+    stream beside the reverse sweep's vector-Jacobian products: variants with
+    smaller register tiles run slower alone but share the CUs with the
+    reverse sweep's kernels.  Times the reverse sweep of an `nsteps`-step
+    adjoint with checkpoint segments of `seg` steps for the plan's own
+    variants and each uniform set, then changes one pass at a time from the
+    best; sets the winner on the plan and returns (variants, {candidate:
+    seconds}).  Call after tune() (the forward sweep keeps the plan's
+    variants)."""
+    count = plan.variant_count()
+    if count == 1 or plan.mhd:
+        return plan.get_recompute_variants(), {}
+    key = _key(plan) + "|recompute"
+    cache = _load() if use_cache else {}
+    if key in cache:
+        plan.set_recompute_variants(cache[key]["variants"])
+        return plan.get_recompute_variants(), cache[key].get("s", {})
+    x0, _ = _probe_state(plan)
+    aws = plan.adjoint_workspace(nsteps, seg)
+    dt, nu = 1e-3, 1e-3
+    seen = {}
+
+    def sweep_s(vs):
+        k = ",".join(map(str, vs))
+        if k not in seen:
+            plan.set_recompute_variants(vs)
+            best = math.inf
+            for _ in range(rounds):
+                plan.adjoint_forward(x0, nsteps, dt, nu, 1, seg, aws)
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                plan.adjoint_backward(nsteps, dt, nu, seg, aws)
+                torch.cuda.synchronize()
+                best = min(best, time.perf_counter() - t0)
+            seen[k] = best
+        return seen[k]
+
+    own = plan.get_variants()
+    sweep_s(own)  # warm-up: the reverse sweep's stream and events are created on first use
+    seen.clear()
+    best = min([own] + [[v] * len(PASS_NAMES) for v in range(count)], key=sweep_s)
+    for p in range(len(PASS_NAMES)):
+        for v in range(count):
+            cand = list(best)
+            cand[p] = v
+            if sweep_s(cand) < sweep_s(best):
+                best = cand
+    del aws
+    plan.set_recompute_variants(best)
+    if verbose:
+        print("autotune recompute", key, best, {k: round(s * 1e3, 2) for k, s in seen.items()})
+    if use_cache:
+        cache = _load()
+        cache[key] = {"variants": best, "s": seen}
+        _store(cache)
+    return best, seen

@@ -58,6 +58,9 @@ struct tg_plan {
   bool generic = false;  // no compiled kernels for n: direct-sum transforms (tg_generic.hip)
   int nDiagBlk, nMax;
   int var[TG_NPASS] = {0, 0, 0, 0, 0};  // tile variant per pass
+  // tile variants of the adjoint recompute that runs beside the reverse
+  // sweep (rvar[0] < 0: the plan's own var)
+  int rvar[TG_NPASS] = {-1, -1, -1, -1, -1};
   // optional per-pass timing: pairs of events around each pass launch
   bool timing = false;
   std::vector<hipEvent_t> evpool;

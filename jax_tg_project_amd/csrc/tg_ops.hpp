@@ -286,13 +286,13 @@ template <typename R, int N>
 // bslot (hydro only): where the inverse-y output (u, w) goes and the z pass
 // reads it, instead of the workspace B (the adjoint keeps it per stage).
 int rhs_passes(const tg_plan* p, const Ptrs<R>& w, const cpx<R>* us, const cpx<R>* bs, int fmode,
-               const StageArgs<R>& a, R lor, hipStream_t s, cpx<R>* bslot = nullptr) {
+               const StageArgs<R>& a, R lor, hipStream_t s, cpx<R>* bslot = nullptr, const int* vars = nullptr) {
   const int cpz = a.cpz ? a.cpz : Geo<N>::KZ;
   using Lc = Launch<R, N>;
   const cpx<R>* tw = static_cast<const cpx<R>*>(p->tw);
   const bool mhd = p->mhd != 0;
   const int fo = mhd ? 6 : 3;
-  const int* v = p->var;
+  const int* v = vars ? vars : p->var;
   TG_TIMED(TG_PASS_INV_X, Lc::inv_x(mhd ? XM_MHD5 : XM_HYDRO5, v[TG_PASS_INV_X], us, bs, w.A, tw, R(p->ks), s, R(1),
                                     cpz));
   cpx<R>* bm = bslot ? bslot : w.B;
@@ -462,6 +462,13 @@ struct Ops {
     L.nblk = Launch<R, N>::adj_x_blocks();
     return L;
   }
+  // Segments of the checkpoint schedule: the first one takes the remainder
+  // of nsteps over seg, every later one seg steps, so the two segments the
+  // forward sweep keeps (the last two) are whole.
+  static int seg_len(int nsteps, int seg, int c) {
+    const int r = nsteps - ((nsteps + seg - 1) / seg - 1) * seg;
+    return c == 0 ? r : seg;
+  }
   // `len` RK4 steps from cur keeping the four stage states of step j in
   // st[4j .. 4j+3] (stage 1 = the step's input state); cur ends as the
   // state after the last step.
@@ -476,8 +483,10 @@ struct Ops {
   // Step j of steps_keeping_stages (after its copy st[0] <- cur): it starts
   // from st[4j]; its last stage writes the next step's input slot directly
   // (cur only after the last step).
+  // vars: tile variants of its passes (nullptr: the plan's).
   static int keep_step(const tg_plan* p, cpx<R>* cur, cpx<R>* st, size_t vstride, int j, int len, double dt,
-                       double nu, void* ws, hipStream_t s, cpx<R>* slots, size_t sstride) {
+                       double nu, void* ws, hipStream_t s, cpx<R>* slots, size_t sstride,
+                       const int* vars = nullptr) {
     Ptrs<R> w = carve<R>(p, ws);
     cpx<R>* S = st + size_t(4 * j) * vstride;
     for (int stg = 0; stg < 4; ++stg) {
@@ -489,7 +498,7 @@ struct Ops {
       a.dt = R(dt); a.nu = R(nu); a.ks = R(p->ks); a.stage = stg;
       a.cpz = kzp<R, N>();
       cpx<R>* slot = slots ? slots + size_t(4 * j + stg) * sstride : nullptr;
-      TG_RET(rhs_passes<R, N>(p, w, a.us, nullptr, FM_HYDRO_STAGE, a, R(0), s, slot));
+      TG_RET(rhs_passes<R, N>(p, w, a.us, nullptr, FM_HYDRO_STAGE, a, R(0), s, slot, vars));
     }
     return TG_OK;
   }
@@ -506,7 +515,7 @@ struct Ops {
     TG_HIP(hipMemcpy2DAsync(L.cur, KZP * e, x0, KZ * e, KZ * e, rows, hipMemcpyDeviceToDevice, s));
     for (int c = 0; c < L.nck; ++c) {
       TG_HIP(hipMemcpyAsync(L.ck + c * L.vstride, L.cur, bytes, hipMemcpyDeviceToDevice, s));
-      const int len = std::min(seg, nsteps - c * seg);
+      const int len = seg_len(nsteps, seg, c);
       if (c < L.nck - 2) TG_RET(rk4_padded(p, L.cur, nullptr, len, dt, nu, 0.0, -1.0, ws, s));
       else if (c == L.nck - 2)  // kept in the second stage buffer: the reverse sweep never recomputes it
         TG_RET(steps_keeping_stages(p, L.cur, L.st2, L.vstride, len, dt, nu, ws, s, L.slots2, L.sstride));
@@ -560,8 +569,11 @@ struct Ops {
       TG_HIP(hipEventRecord(mp->adjev[0], s));  // checkpoints written, nupart cleared
       TG_HIP(hipStreamWaitEvent(s2, mp->adjev[0], 0));
     }
+    // the recompute's tile variants: chosen for running beside the reverse
+    // sweep's kernels (smaller register tiles share the CUs better)
+    const int* rv = p->rvar[0] >= 0 ? p->rvar : p->var;
     for (int c = L.nck - 1; c >= 0; --c) {
-      const int len = std::min(seg, nsteps - c * seg);
+      const int len = seg_len(nsteps, seg, c);
       const bool even = (L.nck - 1 - c) % 2 == 0;
       cpx<R>* stc = even ? L.st : L.st2;
       cpx<R>* slc = even ? L.slots : L.slots2;
@@ -574,7 +586,7 @@ struct Ops {
       const bool evn = (L.nck - c) % 2 == 0;
       cpx<R>* stn = evn ? L.st : L.st2;
       cpx<R>* sln = evn ? L.slots : L.slots2;
-      const int lenn = rec ? std::min(seg, nsteps - (c - 1) * seg) : 0;
+      const int lenn = rec ? seg_len(nsteps, seg, c - 1) : 0;
       int jr = 0;
       if (rec) {
         TG_HIP(hipMemcpyAsync(L.cur2, L.ck + (c - 1) * L.vstride, bytes, hipMemcpyDeviceToDevice, s2));
@@ -582,7 +594,7 @@ struct Ops {
       }
       for (int j = len - 1; j >= 0; --j) {
         for (; jr < lenn && jr * len <= (len - 1 - j) * lenn; ++jr)
-          TG_RET(keep_step(p, L.cur2, stn, L.vstride, jr, lenn, dt, nu, L.ws2, s2, sln, L.sstride));
+          TG_RET(keep_step(p, L.cur2, stn, L.vstride, jr, lenn, dt, nu, L.ws2, s2, sln, L.sstride, rv));
         cpx<R>* S = stc + size_t(4 * j) * L.vstride;
         for (int sidx = 3; sidx >= 0; --sidx) {
           StageArgs<R> a{};
@@ -599,7 +611,7 @@ struct Ops {
         std::swap(lamp, lacc);
       }
       for (; jr < lenn; ++jr)
-        TG_RET(keep_step(p, L.cur2, stn, L.vstride, jr, lenn, dt, nu, L.ws2, s2, sln, L.sstride));
+        TG_RET(keep_step(p, L.cur2, stn, L.vstride, jr, lenn, dt, nu, L.ws2, s2, sln, L.sstride, rv));
       if (rec) TG_HIP(hipEventRecord(ev_rec(c - 1), s2));
       if (c > 1) {
         // the recompute of segment c - 2 reuses this buffer: after this reverse

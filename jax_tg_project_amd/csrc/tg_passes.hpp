@@ -1324,37 +1324,54 @@ struct StageArgs {
 // Adjoint stage bookkeeping of one compact point (the FM_ADJ contract of
 // StageArgs) given w = the forward-x output of the product's VJP through u
 // plus the curl of the field that adds to it (so mu = w - nu k^2 a_s).
-// Every load of the point is issued before its stores; returns the point's
-// term of the nu gradient.
+// adj_point_load issues every load of the point; adj_point_store forms and
+// stores its results and returns the point's term of the nu gradient.  A
+// caller issues a point's loads before any of its stores.
 template <typename R>
-__device__ __forceinline__ double adj_point_w(const StageArgs<R>& a, size_t pt, size_t cs, R k2, int iz,
-                                              const cpx<R> (&w)[3]) {
+struct AdjIn {
   cpx<R> ar[3], lp[3], l0[3], xv[3];
+};
+
+template <typename R>
+__device__ __forceinline__ AdjIn<R> adj_point_load(const StageArgs<R>& a, size_t pt, size_t cs) {
+  AdjIn<R> in;
   const bool uselp = a.first == 1 || !a.last;
 #pragma unroll
-  for (int c = 0; c < 3; ++c) ar[c] = a.a[c * cs + pt];
+  for (int c = 0; c < 3; ++c) in.ar[c] = a.a[c * cs + pt];
 #pragma unroll
-  for (int c = 0; c < 3; ++c) lp[c] = uselp ? a.lamp[c * cs + pt] : cpx<R>{R(0), R(0)};
+  for (int c = 0; c < 3; ++c) in.lp[c] = uselp ? a.lamp[c * cs + pt] : cpx<R>{R(0), R(0)};
 #pragma unroll
-  for (int c = 0; c < 3; ++c) l0[c] = a.first == 0 ? a.lacc[c * cs + pt] : cpx<R>{R(0), R(0)};
+  for (int c = 0; c < 3; ++c) in.l0[c] = a.first == 0 ? a.lacc[c * cs + pt] : cpx<R>{R(0), R(0)};
 #pragma unroll
-  for (int c = 0; c < 3; ++c) xv[c] = a.xs ? a.xs[c * cs + pt] : cpx<R>{R(0), R(0)};
+  for (int c = 0; c < 3; ++c) in.xv[c] = a.xs ? a.xs[c * cs + pt] : cpx<R>{R(0), R(0)};
+  return in;
+}
+
+template <typename R>
+__device__ __forceinline__ double adj_point_store(const StageArgs<R>& a, size_t pt, size_t cs, R k2, int iz,
+                                                  const cpx<R> (&w)[3], const AdjIn<R>& in) {
   const R nk = a.nu * k2;
   const double wk = (iz == 0 ? 1.0 : 2.0) * double(k2);
   double nus = 0.0;
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     const size_t o = c * cs + pt;
-    const cpx<R> as = {a.alpha * ar[c].x, a.alpha * ar[c].y};
+    const cpx<R> as = {a.alpha * in.ar[c].x, a.alpha * in.ar[c].y};
     const cpx<R> mu = {w[c].x - nk * as.x, w[c].y - nk * as.y};
     cpx<R> acc = mu;
-    if (a.first == 1) acc = {lp[c].x + mu.x, lp[c].y + mu.y};
-    else if (a.first == 0) acc = {l0[c].x + mu.x, l0[c].y + mu.y};
-    if (a.xs) nus += wk * (double(as.x) * double(xv[c].x) + double(as.y) * double(xv[c].y));
+    if (a.first == 1) acc = {in.lp[c].x + mu.x, in.lp[c].y + mu.y};
+    else if (a.first == 0) acc = {in.l0[c].x + mu.x, in.l0[c].y + mu.y};
+    if (a.xs) nus += wk * (double(as.x) * double(in.xv[c].x) + double(as.y) * double(in.xv[c].y));
     a.lacc[o] = acc;
-    if (!a.last) a.anext[o] = {a.clam * lp[c].x + a.cmu * mu.x, a.clam * lp[c].y + a.cmu * mu.y};
+    if (!a.last) a.anext[o] = {a.clam * in.lp[c].x + a.cmu * mu.x, a.clam * in.lp[c].y + a.cmu * mu.y};
   }
   return nus;
+}
+
+template <typename R>
+__device__ __forceinline__ double adj_point_w(const StageArgs<R>& a, size_t pt, size_t cs, R k2, int iz,
+                                              const cpx<R> (&w)[3]) {
+  return adj_point_store(a, pt, cs, k2, iz, w, adj_point_load(a, pt, cs));
 }
 
 // The same from the six forward-x outputs: v[0..2] the product's VJP
@@ -1579,6 +1596,28 @@ struct CfgXR {
 template <int N>
 constexpr bool use_xr() { return N >= 128 && (N & (N - 1)) == 0; }
 
+// Epilogue over the E register slots of a thread.  AHEAD: the loads of
+// slot s + 1 are issued before the stores of slot s (loads and stores share
+// one in-order counter, so loads issued after a store also wait for that
+// store); otherwise each slot loads, then stores, holding one slot's
+// operands (fewer registers).
+template <int E, class In, bool AHEAD = true, class Ld, class Fin>
+__device__ __forceinline__ void pipelined_points(const Ld& load, const Fin& finish) {
+  if constexpr (!AHEAD) {
+#pragma unroll
+    for (int sl = 0; sl < E; ++sl) finish(sl, load(sl));
+  } else {
+    In cur = load(0);
+#pragma unroll
+    for (int sl = 0; sl < E; ++sl) {
+      In nxt;
+      if (sl + 1 < E) nxt = load(sl + 1);
+      finish(sl, cur);
+      if (sl + 1 < E) cur = nxt;
+    }
+  }
+}
+
 // Epilogues of the register forward-x pass: hydro or MHD RK4 stage, or the
 // adjoint bookkeeping of one stage VJP (the FM_ADJ epilogue of k_fwd_x).
 enum XRKind { XR_HYDRO = 0, XR_MHD = 1, XR_ADJ = 2 };
@@ -1605,18 +1644,33 @@ struct XRBody {
     const bool colok = izr < G::KZ;
     const int iz = colok ? izr : G::KZ - 1;
     stage_twiddles<R, N>(tws, tw);
+    // First-stage points of field f of `in` (slot j * RAD0 + m holds point
+    // t * NB0 + j + m * SIN0).  Each field's loads are issued while the
+    // previous field is transformed (the barriers between fields wait on LDS
+    // only), so a block pays one load latency instead of one per field.
+    constexpr int RAD0 = FirstRadix<Pl>::value, NB0 = E / RAD0, SIN0 = N / RAD0;
+    auto fetch = [&](int f, cpx<R>(&d)[E]) {
+#pragma unroll
+      for (int j = 0; j < NB0; ++j)
+#pragma unroll
+        for (int m = 0; m < RAD0; ++m) {
+          const int n = t * NB0 + j + m * SIN0;
+          const cpx<R> v = gld<nt_mid<N>()>(&in[((size_t(f) * N + n) * G::K + iy) * kzp<R, N>() + iz]);
+          d[j * RAD0 + m] = colok ? v : cpx<R>{R(0), R(0)};
+        }
+    };
     if constexpr (KIND == XR_ADJ) {
       // w = v + i k x p, folded as each p component leaves its transform: only
       // the three w fields and one transformed field are live at a time
       const R ky = a.ks * R(G::kint(iy)), kz = a.ks * R(iz);
       cpx<R> w[3][E];
+      cpx<R> cur[E], nxt[E];
+      fetch(0, cur);
 #pragma unroll
       for (int f = 0; f < 6; ++f) {
+        if (f + 1 < 6) fetch(f + 1, nxt);
         if (f) __syncthreads();  // the previous field's last LDS reads are done
-        auto ld = [&](int n, int, int) -> cpx<R> {
-          const cpx<R> v = gld<nt_mid<N>()>(&in[((size_t(f) * N + n) * G::K + iy) * kzp<R, N>() + iz]);
-          return colok ? v : cpx<R>{R(0), R(0)};
-        };
+        auto ld = [&](int, int, int slot) -> cpx<R> { return cur[slot]; };
         // (i k x p)_x = i (ky p_z - kz p_y), _y = i (kz p_x - kx p_z), _z = i (kx p_y - ky p_x)
         auto st = [&](int n, int, cpx<R> p, int slot) {
           if (f < 3) {
@@ -1636,6 +1690,8 @@ struct XRBody {
           }
         };
         fft_io<R, N, E, -1, 1, true>(buf, L, L, tws, ld, st);
+#pragma unroll
+        for (int i = 0; i < E; ++i) cur[i] = nxt[i];
       }
       adjoint_epilogue(buf, w, colok, t, iy, iz, a);
       return;
@@ -1643,68 +1699,80 @@ struct XRBody {
     const bool mag = MHD && blockIdx.z != 0;
     const int f0 = mag ? 3 : 0;
     cpx<R> o[FF][E];
+    cpx<R> cur[E], nxt[E];
+    fetch(f0, cur);
 #pragma unroll
     for (int f = 0; f < FF; ++f) {
+      if (f + 1 < FF) fetch(f0 + f + 1, nxt);
       if (f) __syncthreads();  // the previous field's last LDS reads are done
-      auto ld = [&](int n, int, int) -> cpx<R> {
-        const cpx<R> v = gld<nt_mid<N>()>(&in[((size_t(f0 + f) * N + n) * G::K + iy) * kzp<R, N>() + iz]);
-        return colok ? v : cpx<R>{R(0), R(0)};
-      };
+      auto ld = [&](int, int, int slot) -> cpx<R> { return cur[slot]; };
       auto st = [&](int, int, cpx<R> v, int slot) { o[f][slot] = v; };
       fft_io<R, N, E, -1, 1, true>(buf, L, L, tws, ld, st);
+#pragma unroll
+      for (int i = 0; i < E; ++i) cur[i] = nxt[i];
     }
     if (!colok) return;
     const int cpz = a.cpz ? a.cpz : G::KZ;
     const size_t cs = size_t(G::K) * G::K * cpz;
     const R ky = a.ks * R(G::kint(iy)), kz = a.ks * R(iz);
-    // slot j*RL + m holds output n = t*NBL + j + m*SL
+    // The u half (momentum from n or m) or the b half (induction from e) of
+    // every kept output point: the loads of the stage input and the RK4
+    // operands, then the update and its stores.
+    const cpx<R>* xsrc = mag ? a.bs : a.us;
+    const cpx<R>* x0 = mag ? a.b0 : a.u0;
+    const cpx<R>* xacc = mag ? a.bacc : a.uacc;
+    struct PtIn {
+      cpx<R> s[3];
+      RK4In<R> in;
+    };
+    auto load = [&](int sl) -> PtIn {
+      PtIn p;
+      const int n = t * NBL + sl / RL + (sl % RL) * SL;
+      if (G::kept(n)) {
+        const size_t pt = (size_t(G::compact_of(n)) * G::K + iy) * cpz + iz;
 #pragma unroll
-    for (int j = 0; j < NBL; ++j) {
-#pragma unroll
-      for (int m = 0; m < RL; ++m) {
-        const int n = t * NBL + j + m * SL;
-        if (!G::kept(n)) continue;
-        const int ix = G::compact_of(n);
-        const size_t pt = (size_t(ix) * G::K + iy) * cpz + iz;
-        cpx<R> v[3] = {o[0][j * RL + m], o[1][j * RL + m], o[2][j * RL + m]};
-        const R kx = a.ks * R(G::kint(ix));
-        const R k2 = kx * kx + ky * ky + kz * kz;
-        const R k2s = (k2 == R(0)) ? R(1) : k2;
-        if (!mag) {
-          // every load of the u half before its stores (see rk4_load)
-          cpx<R> su[3];
-#pragma unroll
-          for (int c = 0; c < 3; ++c) su[c] = a.us[c * cs + pt];
-          const RK4In<R> inu = rk4_load(a.stage, pt, cs, a.u0, a.uacc);
-          // momentum: r_u = P(n or m) - nu k2 u_s
-          project(kx, ky, kz, k2s, v);
-          cpx<R> ru[3];
-          const R nk = a.nu * k2;
-#pragma unroll
-          for (int c = 0; c < 3; ++c) ru[c] = {v[c].x - nk * su[c].x, v[c].y - nk * su[c].y};
-          rk4_store(a.stage, a.dt, pt, cs, ru, inu, a.unext, a.uacc);
-        }
-        if constexpr (MHD) {
-          if (!mag) continue;
-          // induction: r_b = P(i k x e) - eta k2 b_s (the b half's loads, then its stores)
-          cpx<R> sbv[3];
-#pragma unroll
-          for (int c = 0; c < 3; ++c) sbv[c] = a.bs[c * cs + pt];
-          const RK4In<R> inb = rk4_load(a.stage, pt, cs, a.b0, a.bacc);
-          const cpx<R> e0 = v[0], e1 = v[1], e2 = v[2];
-          cpx<R> cb[3];
-          cb[0] = cmuli(cpx<R>{ky * e2.x - kz * e1.x, ky * e2.y - kz * e1.y});
-          cb[1] = cmuli(cpx<R>{kz * e0.x - kx * e2.x, kz * e0.y - kx * e2.y});
-          cb[2] = cmuli(cpx<R>{kx * e1.x - ky * e0.x, kx * e1.y - ky * e0.y});
-          project(kx, ky, kz, k2s, cb);
-          const R ek = a.eta * k2;
-          cpx<R> rb[3];
-#pragma unroll
-          for (int c = 0; c < 3; ++c) rb[c] = {cb[c].x - ek * sbv[c].x, cb[c].y - ek * sbv[c].y};
-          rk4_store(a.stage, a.dt, pt, cs, rb, inb, a.bnext, a.bacc);
-        }
+        for (int c = 0; c < 3; ++c) p.s[c] = xsrc[c * cs + pt];
+        p.in = rk4_load(a.stage, pt, cs, x0, xacc);
       }
-    }
+      return p;
+    };
+    auto finish = [&](int sl, const PtIn& p) {
+      const int n = t * NBL + sl / RL + (sl % RL) * SL;
+      if (!G::kept(n)) return;
+      const int ix = G::compact_of(n);
+      const size_t pt = (size_t(ix) * G::K + iy) * cpz + iz;
+      cpx<R> v[3] = {o[0][sl], o[1][sl], o[2][sl]};
+      const R kx = a.ks * R(G::kint(ix));
+      const R k2 = kx * kx + ky * ky + kz * kz;
+      const R k2s = (k2 == R(0)) ? R(1) : k2;
+      if (!mag) {
+        // momentum: r_u = P(n or m) - nu k2 u_s
+        project(kx, ky, kz, k2s, v);
+        cpx<R> ru[3];
+        const R nk = a.nu * k2;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) ru[c] = {v[c].x - nk * p.s[c].x, v[c].y - nk * p.s[c].y};
+        rk4_store(a.stage, a.dt, pt, cs, ru, p.in, a.unext, a.uacc);
+      }
+      if constexpr (MHD) {
+        if (!mag) return;
+        // induction: r_b = P(i k x e) - eta k2 b_s
+        const cpx<R> e0 = v[0], e1 = v[1], e2 = v[2];
+        cpx<R> cb[3];
+        cb[0] = cmuli(cpx<R>{ky * e2.x - kz * e1.x, ky * e2.y - kz * e1.y});
+        cb[1] = cmuli(cpx<R>{kz * e0.x - kx * e2.x, kz * e0.y - kx * e2.y});
+        cb[2] = cmuli(cpx<R>{kx * e1.x - ky * e0.x, kx * e1.y - ky * e0.y});
+        project(kx, ky, kz, k2s, cb);
+        const R ek = a.eta * k2;
+        cpx<R> rb[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) rb[c] = {cb[c].x - ek * p.s[c].x, cb[c].y - ek * p.s[c].y};
+        rk4_store(a.stage, a.dt, pt, cs, rb, p.in, a.bnext, a.bacc);
+      }
+    };
+    // FP64 at 8 points per thread has no registers to spare for the next
+    // point's operands under its three-wavefront budget
+    pipelined_points<E, PtIn, !(sizeof(R) == 8 && E >= 8)>(load, finish);
   }
 
   // Stage-VJP bookkeeping on the register-held lines (w = the VJP of the
@@ -1721,20 +1789,22 @@ struct XRBody {
       const int cpz = a.cpz ? a.cpz : G::KZ;
       const size_t cs = size_t(G::K) * G::K * cpz;
       const R ky = a.ks * R(G::kint(iy)), kz = a.ks * R(iz);
-#pragma unroll
-      for (int j = 0; j < NBL; ++j) {
-#pragma unroll
-        for (int m = 0; m < RL; ++m) {
-          const int n = t * NBL + j + m * SL;
-          if (!G::kept(n)) continue;
-          const int ix = G::compact_of(n);
-          const size_t pt = (size_t(ix) * G::K + iy) * cpz + iz;
-          const int sl = j * RL + m;
-          const R kx = a.ks * R(G::kint(ix));
-          const cpx<R> wv[3] = {w[0][sl], w[1][sl], w[2][sl]};
-          nusum += adj_point_w(a, pt, cs, kx * kx + ky * ky + kz * kz, iz, wv);
-        }
-      }
+      auto load = [&](int sl) -> AdjIn<R> {
+        AdjIn<R> p;
+        const int n = t * NBL + sl / RL + (sl % RL) * SL;
+        if (G::kept(n)) p = adj_point_load(a, (size_t(G::compact_of(n)) * G::K + iy) * cpz + iz, cs);
+        return p;
+      };
+      auto finish = [&](int sl, const AdjIn<R>& p) {
+        const int n = t * NBL + sl / RL + (sl % RL) * SL;
+        if (!G::kept(n)) return;
+        const int ix = G::compact_of(n);
+        const R kx = a.ks * R(G::kint(ix));
+        const cpx<R> wv[3] = {w[0][sl], w[1][sl], w[2][sl]};
+        nusum += adj_point_store(a, (size_t(ix) * G::K + iy) * cpz + iz, cs, kx * kx + ky * ky + kz * kz, iz, wv,
+                                 p);
+      };
+      pipelined_points<E, AdjIn<R>>(load, finish);
     }
     if (a.nupart) {
       __syncthreads();  // every thread is done reading buf

@@ -375,6 +375,23 @@ int tg_plan_variants(tg_plan* p, const int* set, int* get, int* count) {
   return TG_OK;
 }
 
+int tg_plan_recompute_variants(tg_plan* p, const int* set, int* get) {
+  TG_RET(check_plan(p));
+  const int nv = (!p->generic && p->n >= 128) ? NVAR : 1;
+  if (set) {
+    if (set[0] == -1) {
+      for (int i = 0; i < TG_NPASS; ++i) p->rvar[i] = -1;
+    } else {
+      for (int i = 0; i < TG_NPASS; ++i)
+        if (set[i] < 0 || set[i] >= nv) return fail(TG_ERR_ARG, "tg_plan_recompute_variants: variant out of range");
+      for (int i = 0; i < TG_NPASS; ++i) p->rvar[i] = set[i];
+    }
+  }
+  if (get)
+    for (int i = 0; i < TG_NPASS; ++i) get[i] = p->rvar[0] >= 0 ? p->rvar[i] : p->var[i];
+  return TG_OK;
+}
+
 int tg_timing_enable(tg_plan* p, int on) {
   TG_RET(check_plan(p));
   p->timing = on != 0;

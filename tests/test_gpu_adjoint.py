@@ -267,3 +267,39 @@ def test_vjp_matches_composed_transforms(N):
     mu = p.vjp(x, a, nu)
     err = (torch.linalg.vector_norm(mu - ref) / torch.linalg.vector_norm(ref)).item()
     assert err < 1e-13, err
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_recompute_variants_change_gradient_by_round_off_only(dtype):
+    """The reverse sweep's recompute may run its own tile variants (chosen by
+    autotune.tune_recompute for running beside the reverse sweep): the
+    gradient then differs from the plan's-own-variants gradient by round-off
+    only.  N = 128, 11 steps in segments of 3 (a short first segment of 2,
+    three recomputed segments), every uniform recompute set and the tuned one."""
+    from jax_tg_project_amd import autotune, ensemble
+    N, nu, dt, n, seg = 128, 1.0 / 400, 5e-3, 11, 3
+    p = _plan(N, dtype)
+    x = p.forward(torch.tensor(ensemble.perturbed_taylor_green(N, 2024), dtype=dtype, device="cuda"))
+    aws = p.adjoint_workspace(n, seg)
+
+    def grad():
+        J = p.adjoint_forward(x, n, dt, nu, 1, seg, aws)
+        g, dnu = p.adjoint_backward(n, dt, nu, seg, aws)
+        return J, g.clone(), dnu
+
+    p.set_recompute_variants(None)
+    assert p.get_recompute_variants() == p.get_variants()
+    J0, g0, n0 = grad()
+    tol = 1e-12 if dtype == torch.float64 else 2e-5
+    for vs in ([0] * 5, [1] * 5, [2] * 5, [1, 2, 1, 2, 0]):
+        p.set_recompute_variants(vs)
+        assert p.get_recompute_variants() == vs
+        J, g, dnu = grad()
+        assert J == J0  # the forward sweep keeps the plan's variants
+        err = (torch.linalg.vector_norm(g - g0) / torch.linalg.vector_norm(g0)).item()
+        assert err <= tol, (vs, err)
+        assert abs(dnu - n0) <= tol * abs(n0), (vs, dnu, n0)
+    best, seen = autotune.tune_recompute(p, nsteps=12, seg=3, use_cache=False)
+    assert p.get_recompute_variants() == best and len(seen) >= 4
+    with pytest.raises(Exception):
+        p.set_recompute_variants([3, 0, 0, 0, 0])

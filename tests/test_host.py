@@ -48,6 +48,7 @@ def test_abi_host_side_errors_without_gpu():
     assert lib.tg_plan_create(64, 16, 0, 2 * math.pi, ctypes.byref(h)) == 1  # TG_ERR_ARG
     assert lib.tg_plan_create(64, 64, 0, -1.0, ctypes.byref(h)) == 1
     assert lib.tg_rk4_steps(None, None, None, 1, 0.1, 0.1, 0.0, -1.0, None, None) == 1
+    assert lib.tg_plan_recompute_variants(None, None, None) == 1
     assert lib.tg_plan_destroy(None) == 0
 
 
