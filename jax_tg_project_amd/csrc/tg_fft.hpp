@@ -396,6 +396,21 @@ __device__ __forceinline__ void fft_io(cpx<R>* buf, int LP, int nlines, const cp
   io_apply<R, N, E, DIR, SRC, DST_GLOBAL, PS>(buf, LP, l, t, active, tw, ld, st, typename PlanFor<N, E>::type{});
 }
 
+// The same with an explicit radix plan (e.g. the reverse of PlanFor, whose
+// first stage reads the points in the order the forward plan's last stage
+// leaves them in registers).
+template <typename R, int N, int E, int DIR, int SRC, bool DST_GLOBAL, class Plan, bool TFAST = false, int PS = 0,
+          class Ld, class St>
+__device__ __forceinline__ void fft_io_plan(cpx<R>* buf, int LP, int nlines, const cpx<R>* __restrict__ tw,
+                                            const Ld& ld, const St& st) {
+  const int tid = threadIdx.x;
+  constexpr int TPL = N / E;
+  const int l = TFAST ? tid / TPL : tid % nlines;
+  const int t = TFAST ? tid % TPL : tid / nlines;
+  const bool active = TFAST ? (l < nlines) : (t < TPL);
+  io_apply<R, N, E, DIR, SRC, DST_GLOBAL, PS>(buf, LP, l, t, active, tw, ld, st, Plan{});
+}
+
 // ---------------------------------------------------------------- registers
 // Register-resident Stockham transform for lines owned by lanes of a single
 // wavefront.  The TPL = N / E lanes of a line keep its points in registers;

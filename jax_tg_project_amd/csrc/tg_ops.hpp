@@ -387,15 +387,18 @@ struct Ops {
   // One vector-Jacobian product of the hydro RHS at x for the cotangent a
   // (scaled by a.alpha), with the epilogue bookkeeping described in `a`.
   // uw: the stage's kept inverse-y output (u, w) of x, if any: then only the
-  // cotangent's projection q is transformed.
+  // cotangent's projection q is transformed.  q_in_A: the previous stage
+  // VJP's forward x already left q's inverse-x output in w.A (a.qout of that
+  // launch), so the first pass is skipped.
   static int vjp_passes(const tg_plan* p, const Ptrs<R>& w, const cpx<R>* x, const StageArgs<R>& a,
-                        hipStream_t s, const cpx<R>* uw = nullptr) {
+                        hipStream_t s, const cpx<R>* uw = nullptr, bool q_in_A = false) {
     using Lc = Launch<R, N>;
     const cpx<R>* tw = static_cast<const cpx<R>*>(p->tw);
     const int cpz = a.cpz ? a.cpz : Geo<N>::KZ;  // row pitch of x, a and the epilogue vectors
     if (uw) {
-      TG_TIMED(TG_PASS_INV_X, Lc::inv_x(XM_ADJQ, p->var[TG_PASS_INV_X], a.a, nullptr, w.A, tw, R(p->ks), s, a.alpha,
-                                        cpz));
+      if (!q_in_A)
+        TG_TIMED(TG_PASS_INV_X, Lc::inv_x(XM_ADJQ, p->var[TG_PASS_INV_X], a.a, nullptr, w.A, tw, R(p->ks), s,
+                                          a.alpha, cpz));
       TG_TIMED(TG_PASS_INV_Y, Lc::y(+1, p->var[TG_PASS_INV_Y], 3, w.A, w.B, tw, s));
       TG_TIMED(TG_PASS_Z, Lc::z_adj(uw, w.B, w.A, tw, s));
       TG_TIMED(TG_PASS_FWD_Y, Lc::y(-1, p->var[TG_PASS_FWD_Y], 6, w.A, w.B, tw, s));
@@ -572,6 +575,10 @@ struct Ops {
     // the recompute's tile variants: chosen for running beside the reverse
     // sweep's kernels (smaller register tiles share the CUs better)
     const int* rv = p->rvar[0] >= 0 ? p->rvar : p->var;
+    // On grids with the register forward x, every stage VJP but the last
+    // also does the next one's inverse x of q (StageArgs::qout), so only the
+    // first stage VJP of the sweep runs that pass on its own.
+    bool q_in_A = false;
     for (int c = L.nck - 1; c >= 0; --c) {
       const int len = seg_len(nsteps, seg, c);
       const bool even = (L.nck - 1 - c) % 2 == 0;
@@ -606,7 +613,13 @@ struct Ops {
           a.last = sidx == 0;
           if (sidx > 0) { a.anext = L.abuf; a.clam = cl[sidx - 1]; a.cmu = cm[sidx]; }
           const cpx<R>* uw = slc ? slc + size_t(4 * j + sidx) * L.sstride : nullptr;
-          TG_RET(vjp_passes(p, w, a.xs, a, s, uw));
+          const bool fuse = use_xr<N>() && uw && !(c == 0 && j == 0 && sidx == 0);
+          if (fuse) {
+            a.qout = w.A;
+            a.qalpha = sidx > 0 ? R(1) : cl[3];  // the next VJP's cotangent: abuf, or lamp of the step before
+          }
+          TG_RET(vjp_passes(p, w, a.xs, a, s, uw, q_in_A));
+          q_in_A = fuse;
         }
         std::swap(lamp, lacc);
       }

@@ -1319,6 +1319,12 @@ struct StageArgs {
   R alpha, clam, cmu;
   int first, last;
   double* nupart;
+  // register forward-x only: when qout is set, the pass also leaves the
+  // next stage VJP's first pass done: q = P(qalpha a_next) (a_next = the
+  // lacc it stores when `last`, else anext) inverse-transformed along x
+  // into qout, laid out as k_inv_x<XM_ADJQ> writes it
+  cpx<R>* qout;
+  R qalpha;
 };
 
 // Adjoint stage bookkeeping of one compact point (the FM_ADJ contract of
@@ -1347,9 +1353,12 @@ __device__ __forceinline__ AdjIn<R> adj_point_load(const StageArgs<R>& a, size_t
   return in;
 }
 
+// an (if not null) receives the next stage's cotangent at the point: the
+// lacc it stores when `last`, else anext.
 template <typename R>
 __device__ __forceinline__ double adj_point_store(const StageArgs<R>& a, size_t pt, size_t cs, R k2, int iz,
-                                                  const cpx<R> (&w)[3], const AdjIn<R>& in) {
+                                                  const cpx<R> (&w)[3], const AdjIn<R>& in,
+                                                  cpx<R>* an = nullptr) {
   const R nk = a.nu * k2;
   const double wk = (iz == 0 ? 1.0 : 2.0) * double(k2);
   double nus = 0.0;
@@ -1363,7 +1372,9 @@ __device__ __forceinline__ double adj_point_store(const StageArgs<R>& a, size_t 
     else if (a.first == 0) acc = {in.l0[c].x + mu.x, in.l0[c].y + mu.y};
     if (a.xs) nus += wk * (double(as.x) * double(in.xv[c].x) + double(as.y) * double(in.xv[c].y));
     a.lacc[o] = acc;
-    if (!a.last) a.anext[o] = {a.clam * in.lp[c].x + a.cmu * mu.x, a.clam * in.lp[c].y + a.cmu * mu.y};
+    const cpx<R> nx = {a.clam * in.lp[c].x + a.cmu * mu.x, a.clam * in.lp[c].y + a.cmu * mu.y};
+    if (!a.last) a.anext[o] = nx;
+    if (an) an[c] = a.last ? acc : nx;
   }
   return nus;
 }
@@ -1693,7 +1704,23 @@ struct XRBody {
 #pragma unroll
         for (int i = 0; i < E; ++i) cur[i] = nxt[i];
       }
-      adjoint_epilogue(buf, w, colok, t, iy, iz, a);
+      cpx<R> qn[3][E];
+      adjoint_epilogue(buf, w, colok, t, iy, iz, a, qn);
+      if (a.qout) {
+        // the next stage VJP's inverse x of q: the reversed plan's first stage
+        // reads the points in the slots the epilogue left them in
+        using IP = ReversePlan<Pl>;
+#pragma unroll
+        for (int f = 0; f < 3; ++f) {
+          __syncthreads();  // buf is free (previous transform, nu reduction)
+          auto ld = [&](int, int, int slot) -> cpx<R> { return qn[f][slot]; };
+          auto st = [&](int n, int ll, cpx<R> v, int) {
+            const int izw = iz0 + ll;
+            if (izw < G::KZ) gst<nt_mid<N>()>(&a.qout[((size_t(f) * N + n) * G::K + iy) * kzp<R, N>() + izw], v);
+          };
+          fft_io_plan<R, N, E, +1, 1, true, IP>(buf, L, L, tws, ld, st);
+        }
+      }
       return;
     }
     const bool mag = MHD && blockIdx.z != 0;
@@ -1779,16 +1806,24 @@ struct XRBody {
   // product through u plus the curl of the field that adds to it), as the
   // FM_ADJ epilogue of XFBody states it; the block's share of the nu
   // gradient goes to a.nupart[block].  Every thread reaches the barriers.
+  // qn: the next stage's projected cotangent q = P(qalpha a_next) / N^3 at
+  // each slot (zero off the kept kx), filled when a.qout is set.
   __device__ __forceinline__ static void adjoint_epilogue(cpx<R>* buf, const cpx<R> (&w)[3][C::E], bool colok,
-                                                          int t, int iy, int iz, const StageArgs<R>& a) {
+                                                          int t, int iy, int iz, const StageArgs<R>& a,
+                                                          cpx<R> (&qn)[3][C::E]) {
     constexpr int E = C::E;
     using Pl = typename PlanFor<N, E>::type;
     constexpr int RL = LastRadix<Pl>::value, NBL = E / RL, SL = N / RL;
     double nusum = 0.0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int sl = 0; sl < E; ++sl) qn[c][sl] = {R(0), R(0)};
     if (colok) {
       const int cpz = a.cpz ? a.cpz : G::KZ;
       const size_t cs = size_t(G::K) * G::K * cpz;
       const R ky = a.ks * R(G::kint(iy)), kz = a.ks * R(iz);
+      const R qs = a.qalpha / (R(N) * R(N) * R(N));
       auto load = [&](int sl) -> AdjIn<R> {
         AdjIn<R> p;
         const int n = t * NBL + sl / RL + (sl % RL) * SL;
@@ -1801,10 +1836,20 @@ struct XRBody {
         const int ix = G::compact_of(n);
         const R kx = a.ks * R(G::kint(ix));
         const cpx<R> wv[3] = {w[0][sl], w[1][sl], w[2][sl]};
-        nusum += adj_point_store(a, (size_t(ix) * G::K + iy) * cpz + iz, cs, kx * kx + ky * ky + kz * kz, iz, wv,
-                                 p);
+        const R k2 = kx * kx + ky * ky + kz * kz;
+        cpx<R> an[3];
+        nusum += adj_point_store(a, (size_t(ix) * G::K + iy) * cpz + iz, cs, k2, iz, wv, p, an);
+        if (a.qout) {
+          // q = P(qalpha a_next) / N^3, formed as k_inv_x<XM_ADJQ> forms it
+          const R r = R(1) / ((k2 == R(0)) ? R(1) : k2);
+          const cpx<R> kd = {(kx * an[0].x + ky * an[1].x + kz * an[2].x) * r,
+                             (kx * an[0].y + ky * an[1].y + kz * an[2].y) * r};
+          const R kc[3] = {kx, ky, kz};
+#pragma unroll
+          for (int c = 0; c < 3; ++c) qn[c][sl] = {qs * (an[c].x - kc[c] * kd.x), qs * (an[c].y - kc[c] * kd.y)};
+        }
       };
-      pipelined_points<E, AdjIn<R>>(load, finish);
+      pipelined_points<E, AdjIn<R>, !(sizeof(R) == 8 && E >= 8)>(load, finish);
     }
     if (a.nupart) {
       __syncthreads();  // every thread is done reading buf
