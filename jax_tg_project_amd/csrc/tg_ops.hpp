@@ -104,6 +104,17 @@ struct Launch {
       return TG_OK;
     });
   }
+  // Forward y of the adjoint z pass's kz-folded fields into the five fields
+  // of the register forward x (k_y_adj).
+  static int y_adj(int var, const cpx<R>* in, cpx<R>* out, const cpx<R>* tw, R ks, hipStream_t s) {
+    return with_variant<N>(var, [&](auto vc) {
+      constexpr int V = decltype(vc)::value;
+      using C = CfgXY<R, N, V>;
+      hipLaunchKernelGGL((k_y_adj<R, N, V>), dim3(cdiv(G::KZ, C::L), N, 4), dim3(C::NT), 0, s, in, out, tw, ks);
+      TG_LAUNCHED();
+      return TG_OK;
+    });
+  }
   // Inverse y of `nvec` vectors handed over as five fields each (HYDRO5 /
   // MHD5 x pass), writing (v, i k x v) per vector.
   static int y_curl(int var, int nvec, const cpx<R>* in, cpx<R>* out, const cpx<R>* tw, R ks, hipStream_t s) {
@@ -187,11 +198,18 @@ struct Launch {
   }
   // Register-resident adjoint z pass: u, w from fields 0-5 of `in`, q from
   // fields 0-2 of `qin`.
-  static int z_adj(const cpx<R>* in, const cpx<R>* qin, cpx<R>* out, const cpx<R>* tw, hipStream_t s) {
+  // fold: store the kz-folded fields k_y_adj reads (see ZWAdjBody).
+  static int z_adj(const cpx<R>* in, const cpx<R>* qin, cpx<R>* out, const cpx<R>* tw, hipStream_t s,
+                   bool fold = false, R ks = R(1)) {
     if constexpr (adj_reg_z<N>()) {
       constexpr int WPB = ZwDefault<N>::WPB;
       using C = CfgZW<R, N, adj_ze<R>(), WPB>;
-      hipLaunchKernelGGL((k_zw_adj<R, N, WPB>), dim3(N * N / C::LINES), dim3(C::NT), 0, s, in, qin, out, tw);
+      if (fold)
+        hipLaunchKernelGGL((k_zw_adj<R, N, WPB, true>), dim3(N * N / C::LINES), dim3(C::NT), 0, s, in, qin, out, tw,
+                           ks);
+      else
+        hipLaunchKernelGGL((k_zw_adj<R, N, WPB, false>), dim3(N * N / C::LINES), dim3(C::NT), 0, s, in, qin, out,
+                           tw, ks);
       TG_LAUNCHED();
       return TG_OK;
     } else {
@@ -203,9 +221,13 @@ struct Launch {
   // is the register-epilogue kernel; so is every adjoint launch on those grids.
   static int fwd_x(int mode, int var, const cpx<R>* in, const StageArgs<R>& a, const cpx<R>* tw, hipStream_t s) {
     if constexpr (use_xr<N>()) {
-      if (mode == FM_ADJ) {
+      if (mode == FM_ADJ || mode == FM_ADJ5) {
         using C = CfgXR<R, N>;
-        hipLaunchKernelGGL((k_fwd_xr_adj<R, N>), dim3(cdiv(G::KZ, C::L), G::K), dim3(C::NT), 0, s, in, a, tw);
+        if (mode == FM_ADJ5)
+          hipLaunchKernelGGL((k_fwd_xr_adj<R, N, true>), dim3(cdiv(G::KZ, C::L), G::K), dim3(C::NT), 0, s, in, a, tw);
+        else
+          hipLaunchKernelGGL((k_fwd_xr_adj<R, N, false>), dim3(cdiv(G::KZ, C::L), G::K), dim3(C::NT), 0, s, in, a,
+                             tw);
         TG_LAUNCHED();
         return TG_OK;
       }
@@ -400,9 +422,17 @@ struct Ops {
         TG_TIMED(TG_PASS_INV_X, Lc::inv_x(XM_ADJQ, p->var[TG_PASS_INV_X], a.a, nullptr, w.A, tw, R(p->ks), s,
                                           a.alpha, cpz));
       TG_TIMED(TG_PASS_INV_Y, Lc::y(+1, p->var[TG_PASS_INV_Y], 3, w.A, w.B, tw, s));
-      TG_TIMED(TG_PASS_Z, Lc::z_adj(uw, w.B, w.A, tw, s));
-      TG_TIMED(TG_PASS_FWD_Y, Lc::y(-1, p->var[TG_PASS_FWD_Y], 6, w.A, w.B, tw, s));
-      TG_TIMED(TG_PASS_FWD_X, Lc::fwd_x(FM_ADJ, 0, w.B, a, tw, s));
+      if constexpr (use_xr<N>()) {
+        // i k x p2 folded along the way: kz in the z pass, ky in k_y_adj, kx
+        // in the register forward x, which then transforms five fields
+        TG_TIMED(TG_PASS_Z, Lc::z_adj(uw, w.B, w.A, tw, s, true, R(p->ks)));
+        TG_TIMED(TG_PASS_FWD_Y, Lc::y_adj(p->var[TG_PASS_FWD_Y], w.A, w.B, tw, R(p->ks), s));
+        TG_TIMED(TG_PASS_FWD_X, Lc::fwd_x(FM_ADJ5, 0, w.B, a, tw, s));
+      } else {
+        TG_TIMED(TG_PASS_Z, Lc::z_adj(uw, w.B, w.A, tw, s));
+        TG_TIMED(TG_PASS_FWD_Y, Lc::y(-1, p->var[TG_PASS_FWD_Y], 6, w.A, w.B, tw, s));
+        TG_TIMED(TG_PASS_FWD_X, Lc::fwd_x(FM_ADJ, 0, w.B, a, tw, s));
+      }
       return TG_OK;
     }
     // the 8-points-per-thread tile (variant 1, where compiled) keeps the six

@@ -132,7 +132,9 @@ struct CfgXF {
 enum XMode { XM_HYDRO = 0, XM_MHD = 1, XM_COPY3 = 2, XM_DIV = 3, XM_CURL3 = 4, XM_ADJ = 5, XM_ADJQ = 6,
              XM_HYDRO5 = 7, XM_MHD5 = 8, XM_GRAD3 = 9 };
 enum ZMode { ZM_HYDRO = 0, ZM_MHD = 1, ZM_PHYS_OUT3 = 2, ZM_MAXABS = 3, ZM_PHYS_IN3 = 4, ZM_PHYS_OUT1 = 5, ZM_ADJ = 6 };
-enum FMode { FM_HYDRO_STAGE = 0, FM_MHD_STAGE = 1, FM_STORE3 = 2, FM_HYDRO_RHS = 3, FM_MHD_RHS = 4, FM_ADJ = 5 };
+// FM_ADJ5: FM_ADJ from the five fields of k_y_adj (register forward x only)
+enum FMode { FM_HYDRO_STAGE = 0, FM_MHD_STAGE = 1, FM_STORE3 = 2, FM_HYDRO_RHS = 3, FM_MHD_RHS = 4, FM_ADJ = 5,
+             FM_ADJ5 = 6 };
 
 __host__ __device__ constexpr int xmode_fields(int m) {
   return m == XM_HYDRO ? 6 : m == XM_MHD ? 12 : m == XM_ADJ ? 9 : m == XM_HYDRO5 ? 5 : m == XM_MHD5 ? 10
@@ -428,6 +430,68 @@ __global__ __launch_bounds__((CfgXY<R, N, V>::NT)) void k_y(const cpx<R>* __rest
   using B = YBody<R, N, DIR, V>;
   __shared__ cpx<R> sm[B::SMEM];
   B::run(sm, blockIdx.x, blockIdx.y, blockIdx.z, in, out, tw);
+}
+
+// Forward y of the six fields the adjoint z pass leaves with the kz terms of
+// i k x p2 folded (A0 = p1_x - i kz p2_y, A1 = p1_y + i kz p2_x, A2 = p1_z,
+// A3..A5 = p2): folds the ky terms and keeps the K masked rows of five
+// fields, X = A0 + i ky p2_z, Y = A1, Z = A2 - i ky p2_x, p2_y, p2_z (the
+// register forward x adds the kx terms).  Block type bz: 0 -> X and p2_z
+// from two transforms, 1 -> Y, 2 -> Z from two transforms, 3 -> p2_y.
+template <typename R, int N, int V>
+struct YAdjBody {
+  using G = Geo<N>;
+  using C = CfgXY<R, N, V>;
+  static constexpr int SMEM = N * C::L + N;
+  __device__ __forceinline__ static void run(cpx<R>* sm, int bx, int x, int bz, const cpx<R>* __restrict__ in,
+                                             cpx<R>* __restrict__ out, const cpx<R>* __restrict__ tw, R ks) {
+    constexpr int L = C::L, E = C::E;
+    cpx<R>* buf = sm;
+    cpx<R>* tws = sm + N * L;
+    const int iz0 = bx * L;
+    stage_twiddles<R, N>(tws, tw);
+    auto ldf = [&](int f) {
+      return [&, f](int n, int ll, int) -> cpx<R> {
+        const int izr = iz0 + ll;
+        const bool ok = izr < G::KZ;
+        const cpx<R> v = gld<nt_mid<N>()>(&in[((size_t(f) * N + x) * N + n) * kzp<R, N>() + (ok ? izr : G::KZ - 1)]);
+        return ok ? v : cpx<R>{R(0), R(0)};
+      };
+    };
+    auto put = [&](int f, int n, int ll, cpx<R> v) {
+      const int iz = iz0 + ll;
+      if (iz < G::KZ && G::kept(n))
+        gst<nt_mid<N>()>(&out[((size_t(f) * N + x) * G::K + G::compact_of(n)) * kzp<R, N>() + iz], v);
+    };
+    auto kyi = [&](int n, cpx<R> v) -> cpx<R> {  // i ky v at output row n
+      const R ky = ks * R(G::kint(G::kept(n) ? G::compact_of(n) : 0));
+      return cmuli(cpx<R>{ky * v.x, ky * v.y});
+    };
+    if (bz == 1 || bz == 3) {
+      const int f = bz == 1 ? 1 : 4;
+      fft_io<R, N, E, -1, 1, true>(buf, L, L, tws, ldf(f),
+                                   [&](int n, int ll, cpx<R> v, int) { put(bz == 1 ? 1 : 3, n, ll, v); });
+      return;
+    }
+    cpx<R> h[E];  // the first transform (p2_z for bz 0, p2_x for bz 2) by slot
+    fft_io<R, N, E, -1, 1, true>(buf, L, L, tws, ldf(bz == 0 ? 5 : 3), [&](int n, int ll, cpx<R> v, int slot) {
+      h[slot] = v;
+      if (bz == 0) put(4, n, ll, v);
+    });
+    __syncthreads();  // the first transform's last LDS reads are done
+    fft_io<R, N, E, -1, 1, true>(buf, L, L, tws, ldf(bz == 0 ? 0 : 2), [&](int n, int ll, cpx<R> v, int slot) {
+      put(bz == 0 ? 0 : 2, n, ll, bz == 0 ? cadd(v, kyi(n, h[slot])) : csub(v, kyi(n, h[slot])));
+    });
+  }
+};
+
+template <typename R, int N, int V>
+__global__ __launch_bounds__((CfgXY<R, N, V>::NT)) void k_y_adj(const cpx<R>* __restrict__ in,
+                                                              cpx<R>* __restrict__ out,
+                                                              const cpx<R>* __restrict__ tw, R ks) {
+  using B = YAdjBody<R, N, V>;
+  __shared__ cpx<R> sm[B::SMEM];
+  B::run(sm, blockIdx.x, blockIdx.y, blockIdx.z, in, out, tw, ks);
 }
 
 // Inverse y transform that completes the curl.  The x pass (HYDRO5 / MHD5)
@@ -1002,9 +1066,12 @@ __device__ __forceinline__ void zw_combine(cpx<R>* v, const cpx<R>* ra, const cp
 // x (output order of plan FP) into fields FA (real part) and FB: they are
 // split by Hermitian symmetry from the pair (k, N - k), through the line
 // buffer.  Ends with the buffer free again.
-template <typename R, int N, class C, int FA, int FB>
-__device__ __forceinline__ void zw_store_pair(const cpx<R>* x, cpx<R>* lb, cpx<R>* __restrict__ dst, size_t NNKZ,
-                                              int t) {
+// Split the forward transform x of the two real fields a + i b packed in it
+// (output order of plan FP) into their kept kz coefficients, through the
+// line buffer: fn(i, kz, a_kz, b_kz) for the lane's kz = t + i * TPL < KZ.
+// Ends with the buffer free again.
+template <typename R, int N, class C, class Fn>
+__device__ __forceinline__ void zw_split_pair(const cpx<R>* x, cpx<R>* lb, int t, const Fn& fn) {
   using IP = typename C::IP;
   constexpr int E = C::E, TPL = C::TPL, PS = C::PS, KZ = Geo<N>::KZ;
   constexpr int R1 = FirstRadix<IP>::value, NB1 = E / R1, S1 = N / R1;
@@ -1019,11 +1086,22 @@ __device__ __forceinline__ void zw_store_pair(const cpx<R>* x, cpx<R>* lb, cpx<R
     if (kz < KZ) {
       const cpx<R> P = lb[pidx<PS>(kz)];
       const cpx<R> Qc = lb[pidx<PS>(kz == 0 ? 0 : N - kz)];
-      gst<nt_mid<N>()>(&dst[FA * NNKZ + kz], cpx<R>{R(0.5) * (P.x + Qc.x), R(0.5) * (P.y - Qc.y)});
-      gst<nt_mid<N>()>(&dst[FB * NNKZ + kz], cpx<R>{R(0.5) * (P.y + Qc.y), R(0.5) * (Qc.x - P.x)});
+      fn(i, kz, cpx<R>{R(0.5) * (P.x + Qc.x), R(0.5) * (P.y - Qc.y)},
+         cpx<R>{R(0.5) * (P.y + Qc.y), R(0.5) * (Qc.x - P.x)});
     }
   }
   wave_lds_sync();
+}
+
+// Store the kept kz of the two real fields packed in x into fields FA (real
+// part) and FB of dst.
+template <typename R, int N, class C, int FA, int FB>
+__device__ __forceinline__ void zw_store_pair(const cpx<R>* x, cpx<R>* lb, cpx<R>* __restrict__ dst, size_t NNKZ,
+                                              int t) {
+  zw_split_pair<R, N, C>(x, lb, t, [&](int, int kz, cpx<R> a, cpx<R> b) {
+    gst<nt_mid<N>()>(&dst[FA * NNKZ + kz], a);
+    gst<nt_mid<N>()>(&dst[FB * NNKZ + kz], b);
+  });
 }
 
 template <typename R, int N, int E_, int WPB, int PITCH = kzp<R, N>()>
@@ -1102,8 +1180,10 @@ struct ZWBody {
 // five packed sequences (u_x + i u_y), (u_z + i w_x), (w_y + i w_z), (q_x +
 // i q_y), (q_z + 0 i); products p1 = w x q and p2 = q x u; three forward
 // sequences (p1_x + i p1_y), (p1_z + i p2_x), (p2_y + i p2_z) stored as
-// fields 0-5.
-template <typename R, int N, int E_, int WPB>
+// fields 0-5.  FOLD: fields 0 and 1 take the kz terms of i k x p2 as they
+// are stored, p1_x - i kz p2_y and p1_y + i kz p2_x (k_y_adj folds the ky
+// terms, the register forward x the kx terms).
+template <typename R, int N, int E_, int WPB, bool FOLD = false>
 struct ZWAdjBody {
   using G = Geo<N>;
   using C = CfgZW<R, N, E_, WPB>;
@@ -1112,7 +1192,7 @@ struct ZWAdjBody {
   static constexpr int SMEM = C::SMEM;
   __device__ __forceinline__ static void run(cpx<R>* sm, int blk, const cpx<R>* __restrict__ in,
                                              const cpx<R>* __restrict__ qin, cpx<R>* __restrict__ out,
-                                             const cpx<R>* __restrict__ tw) {
+                                             const cpx<R>* __restrict__ tw, R ks = R(1)) {
     constexpr int E = C::E, TPL = C::TPL, LW = C::LW, PS = C::PS;
     constexpr int KZP = kzp<R, N>();
     const size_t NNKZ = size_t(N) * N * KZP;
@@ -1158,9 +1238,29 @@ struct ZWAdjBody {
     }
 #pragma unroll
     for (int p = 0; p < 3; ++p) reg_fft<R, N, E, -1, PS>(f[p], t, lb, tsf, FP{});
-    zw_store_pair<R, N, C, 0, 1>(f[0], lb, dst, NNKZ, t);
-    zw_store_pair<R, N, C, 2, 3>(f[1], lb, dst, NNKZ, t);
-    zw_store_pair<R, N, C, 4, 5>(f[2], lb, dst, NNKZ, t);
+    if constexpr (FOLD) {
+      constexpr int NKZ = (G::KZ + TPL - 1) / TPL;
+      cpx<R> px[NKZ], py[NKZ];  // p2_x, p2_y at the lane's kz
+      zw_split_pair<R, N, C>(f[2], lb, t, [&](int i, int kz, cpx<R> a, cpx<R> b) {
+        py[i] = a;
+        gst<nt_mid<N>()>(&dst[4 * NNKZ + kz], a);
+        gst<nt_mid<N>()>(&dst[5 * NNKZ + kz], b);
+      });
+      zw_split_pair<R, N, C>(f[1], lb, t, [&](int i, int kz, cpx<R> a, cpx<R> b) {
+        px[i] = b;
+        gst<nt_mid<N>()>(&dst[2 * NNKZ + kz], a);
+        gst<nt_mid<N>()>(&dst[3 * NNKZ + kz], b);
+      });
+      zw_split_pair<R, N, C>(f[0], lb, t, [&](int i, int kz, cpx<R> a, cpx<R> b) {
+        const R k = ks * R(kz);
+        gst<nt_mid<N>()>(&dst[kz], cpx<R>{a.x + k * py[i].y, a.y - k * py[i].x});      // p1_x - i kz p2_y
+        gst<nt_mid<N>()>(&dst[NNKZ + kz], cpx<R>{b.x - k * px[i].y, b.y + k * px[i].x});  // p1_y + i kz p2_x
+      });
+    } else {
+      zw_store_pair<R, N, C, 0, 1>(f[0], lb, dst, NNKZ, t);
+      zw_store_pair<R, N, C, 2, 3>(f[1], lb, dst, NNKZ, t);
+      zw_store_pair<R, N, C, 4, 5>(f[2], lb, dst, NNKZ, t);
+    }
   }
 };
 
@@ -1280,13 +1380,13 @@ void k_zw_mhd(const cpx<R>* __restrict__ in, cpx<R>* __restrict__ out, const cpx
 template <typename R>
 constexpr int adj_ze() { return sizeof(R) == 8 ? 8 : 16; }
 
-template <typename R, int N, int WPB>
+template <typename R, int N, int WPB, bool FOLD>
 __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(2)))
 void k_zw_adj(const cpx<R>* __restrict__ in, const cpx<R>* __restrict__ qin, cpx<R>* __restrict__ out,
-              const cpx<R>* __restrict__ tw) {
-  using B = ZWAdjBody<R, N, adj_ze<R>(), WPB>;
+              const cpx<R>* __restrict__ tw, R ks) {
+  using B = ZWAdjBody<R, N, adj_ze<R>(), WPB, FOLD>;
   __shared__ cpx<R> sm[B::SMEM];
-  B::run(sm, blockIdx.x, in, qin, out, tw);
+  B::run(sm, blockIdx.x, in, qin, out, tw, ks);
 }
 
 // ----------------------------------------------------------------- pass 5
@@ -1326,6 +1426,28 @@ struct StageArgs {
   cpx<R>* qout;
   R qalpha;
 };
+
+// Sum of v over the block, returned to thread 0 in a fixed order (the
+// result is reproducible): a butterfly over each wavefront, then thread 0
+// adds the wavefront sums; blocks that are not whole wavefronts add every
+// thread's value in turn.  red: one double per thread of LDS, free on entry.
+// Every thread must call it.
+__device__ __forceinline__ double block_sum(double v, double* red) {
+  const int nt = int(blockDim.x);
+  const bool waves = nt % 64 == 0;
+  if (waves) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  } else {
+    red[threadIdx.x] = v;
+  }
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < (waves ? nt / 64 : nt); ++i) s += red[i];
+  return s;
+}
 
 // Adjoint stage bookkeeping of one compact point (the FM_ADJ contract of
 // StageArgs) given w = the forward-x output of the product's VJP through u
@@ -1563,12 +1685,8 @@ struct XFBody {
   if constexpr (MODE == FM_ADJ) {
     if (a.nupart) {
       __syncthreads();  // every thread is done reading buf
-      double* red = reinterpret_cast<double*>(buf);
-      red[threadIdx.x] = nusum;
-      __syncthreads();
+      const double t = block_sum(nusum, reinterpret_cast<double*>(buf));
       if (threadIdx.x == 0) {
-        double t = 0.0;
-        for (int i = 0; i < int(blockDim.x); ++i) t += red[i];
         a.nupart[size_t(by) * gridDim.x + bx] += t;
       }
     }
@@ -1631,7 +1749,9 @@ __device__ __forceinline__ void pipelined_points(const Ld& load, const Fin& fini
 
 // Epilogues of the register forward-x pass: hydro or MHD RK4 stage, or the
 // adjoint bookkeeping of one stage VJP (the FM_ADJ epilogue of k_fwd_x).
-enum XRKind { XR_HYDRO = 0, XR_MHD = 1, XR_ADJ = 2 };
+// XR_ADJ5: the adjoint from the five fields of k_y_adj (only the kx terms
+// of i k x p2 left to fold).
+enum XRKind { XR_HYDRO = 0, XR_MHD = 1, XR_ADJ = 2, XR_ADJ5 = 3 };
 
 template <typename R, int N, int KIND = XR_HYDRO>
 struct XRBody {
@@ -1670,16 +1790,17 @@ struct XRBody {
           d[j * RAD0 + m] = colok ? v : cpx<R>{R(0), R(0)};
         }
     };
-    if constexpr (KIND == XR_ADJ) {
+    if constexpr (KIND == XR_ADJ || KIND == XR_ADJ5) {
       // w = v + i k x p, folded as each p component leaves its transform: only
       // the three w fields and one transformed field are live at a time
+      constexpr int NF = KIND == XR_ADJ5 ? 5 : 6;
       const R ky = a.ks * R(G::kint(iy)), kz = a.ks * R(iz);
       cpx<R> w[3][E];
       cpx<R> cur[E], nxt[E];
       fetch(0, cur);
 #pragma unroll
-      for (int f = 0; f < 6; ++f) {
-        if (f + 1 < 6) fetch(f + 1, nxt);
+      for (int f = 0; f < NF; ++f) {
+        if (f + 1 < NF) fetch(f + 1, nxt);
         if (f) __syncthreads();  // the previous field's last LDS reads are done
         auto ld = [&](int, int, int slot) -> cpx<R> { return cur[slot]; };
         // (i k x p)_x = i (ky p_z - kz p_y), _y = i (kz p_x - kx p_z), _z = i (kx p_y - ky p_x)
@@ -1689,6 +1810,12 @@ struct XRBody {
             return;
           }
           const R kx = a.ks * R(G::kint(G::kept(n) ? G::compact_of(n) : 0));
+          if constexpr (KIND == XR_ADJ5) {
+            // fields 3, 4: p2_y -> w_z += i kx p2_y, p2_z -> w_y -= i kx p2_z
+            if (f == 3) w[2][slot] = cadd(w[2][slot], cmuli(cpx<R>{kx * p.x, kx * p.y}));
+            else w[1][slot] = csub(w[1][slot], cmuli(cpx<R>{kx * p.x, kx * p.y}));
+            return;
+          }
           if (f == 3) {
             w[1][slot] = cadd(w[1][slot], cmuli(cpx<R>{kz * p.x, kz * p.y}));
             w[2][slot] = csub(w[2][slot], cmuli(cpx<R>{ky * p.x, ky * p.y}));
@@ -1853,14 +1980,8 @@ struct XRBody {
     }
     if (a.nupart) {
       __syncthreads();  // every thread is done reading buf
-      double* red = reinterpret_cast<double*>(buf);
-      red[threadIdx.x] = nusum;
-      __syncthreads();
-      if (threadIdx.x == 0) {
-        double s = 0.0;
-        for (int i = 0; i < int(blockDim.x); ++i) s += red[i];
-        a.nupart[size_t(blockIdx.y) * gridDim.x + blockIdx.x] += s;
-      }
+      const double s = block_sum(nusum, reinterpret_cast<double*>(buf));
+      if (threadIdx.x == 0) a.nupart[size_t(blockIdx.y) * gridDim.x + blockIdx.x] += s;
     }
   }
 };
@@ -1871,7 +1992,7 @@ struct XRBody {
 // allocation (capping them spills more than it gains).
 template <typename R, int KIND>
 constexpr int xr_waves_per_eu() {
-  return (sizeof(R) == 8 && KIND != XR_ADJ) ? 3 : (KIND == XR_ADJ ? 2 : 1);
+  return (KIND == XR_ADJ || KIND == XR_ADJ5) ? 2 : (sizeof(R) == 8 ? 3 : 1);
 }
 
 template <typename R, int N, bool MHD>
@@ -1885,11 +2006,11 @@ __attribute__((amdgpu_waves_per_eu(xr_waves_per_eu<R, MHD ? XR_MHD : XR_HYDRO>()
 
 // Forward x of one stage VJP with its adjoint epilogue on register-held
 // lines (grids of N >= 128; one 128-byte line of kz columns per block).
-template <typename R, int N>
+template <typename R, int N, bool FOLDED>
 __global__ __launch_bounds__((CfgXR<R, N>::NT)) __attribute__((amdgpu_waves_per_eu(xr_waves_per_eu<R, XR_ADJ>())))
 void k_fwd_xr_adj(const cpx<R>* __restrict__ in, StageArgs<R> a,
                                                                  const cpx<R>* __restrict__ tw) {
-  using B = XRBody<R, N, XR_ADJ>;
+  using B = XRBody<R, N, FOLDED ? XR_ADJ5 : XR_ADJ>;
   __shared__ cpx<R> sm[B::SMEM];
   B::run(sm, blockIdx.x, blockIdx.y, in, a, tw);
 }
