@@ -1797,10 +1797,16 @@ struct XRBody {
       const R ky = a.ks * R(G::kint(iy)), kz = a.ks * R(iz);
       cpx<R> w[3][E];
       cpx<R> cur[E], nxt[E];
+      // FP64 at 8 points per lane keeps no registers for the next field
+      constexpr bool PREF = !(sizeof(R) == 8 && E >= 8);
       fetch(0, cur);
 #pragma unroll
       for (int f = 0; f < NF; ++f) {
-        if (f + 1 < NF) fetch(f + 1, nxt);
+        if constexpr (PREF) {
+          if (f + 1 < NF) fetch(f + 1, nxt);
+        } else {
+          if (f) fetch(f, cur);
+        }
         if (f) __syncthreads();  // the previous field's last LDS reads are done
         auto ld = [&](int, int, int slot) -> cpx<R> { return cur[slot]; };
         // (i k x p)_x = i (ky p_z - kz p_y), _y = i (kz p_x - kx p_z), _z = i (kx p_y - ky p_x)
@@ -1828,8 +1834,10 @@ struct XRBody {
           }
         };
         fft_io<R, N, E, -1, 1, true>(buf, L, L, tws, ld, st);
+        if constexpr (PREF) {
 #pragma unroll
-        for (int i = 0; i < E; ++i) cur[i] = nxt[i];
+          for (int i = 0; i < E; ++i) cur[i] = nxt[i];
+        }
       }
       cpx<R> qn[3][E];
       adjoint_epilogue(buf, w, colok, t, iy, iz, a, qn);
@@ -1924,9 +1932,10 @@ struct XRBody {
         rk4_store(a.stage, a.dt, pt, cs, rb, p.in, a.bnext, a.bacc);
       }
     };
-    // FP64 at 8 points per thread has no registers to spare for the next
-    // point's operands under its three-wavefront budget
-    pipelined_points<E, PtIn, !(sizeof(R) == 8 && E >= 8)>(load, finish);
+    // at 8 points per thread the next point's operands cost more in
+    // registers (FP64: spills under its three-wavefront budget; FP32: a
+    // wavefront per SIMD) than the overlap gains (measured at 256^3)
+    pipelined_points<E, PtIn, (E < 8)>(load, finish);
   }
 
   // Stage-VJP bookkeeping on the register-held lines (w = the VJP of the
@@ -1976,7 +1985,7 @@ struct XRBody {
           for (int c = 0; c < 3; ++c) qn[c][sl] = {qs * (an[c].x - kc[c] * kd.x), qs * (an[c].y - kc[c] * kd.y)};
         }
       };
-      pipelined_points<E, AdjIn<R>, !(sizeof(R) == 8 && E >= 8)>(load, finish);
+      pipelined_points<E, AdjIn<R>, (E < 8)>(load, finish);
     }
     if (a.nupart) {
       __syncthreads();  // every thread is done reading buf
