@@ -113,7 +113,9 @@ def tune_recompute(plan, nsteps=64, seg=8, use_cache=True, verbose=False, rounds
     variants and each uniform set, then changes one pass at a time from the
     best; sets the winner on the plan and returns (variants, {candidate:
     seconds}).  Call after tune() (the forward sweep keeps the plan's
-    variants)."""
+    variants).  The trial halves seg (and nsteps with it) until its
+    workspace fits a quarter of the free device memory, and tunes nothing
+    when even segments of 2 steps do not fit."""
     count = plan.variant_count()
     if count == 1 or plan.mhd:
         return plan.get_recompute_variants(), {}
@@ -122,6 +124,11 @@ def tune_recompute(plan, nsteps=64, seg=8, use_cache=True, verbose=False, rounds
     if key in cache:
         plan.set_recompute_variants(cache[key]["variants"])
         return plan.get_recompute_variants(), cache[key].get("s", {})
+    budget = torch.cuda.mem_get_info(plan.device)[0] // 4
+    while plan.adjoint_workspace_bytes(nsteps, seg) > budget and seg > 2:
+        nsteps, seg = nsteps // 2, seg // 2
+    if plan.adjoint_workspace_bytes(nsteps, seg) > budget:
+        return plan.get_recompute_variants(), {}
     x0, _ = _probe_state(plan)
     aws = plan.adjoint_workspace(nsteps, seg)
     dt, nu = 1e-3, 1e-3
