@@ -222,7 +222,7 @@ struct Launch {
   static int fwd_x(int mode, int var, const cpx<R>* in, const StageArgs<R>& a, const cpx<R>* tw, hipStream_t s) {
     if constexpr (use_xr<N>()) {
       if (mode == FM_ADJ || mode == FM_ADJ5) {
-        using C = CfgXR<R, N>;
+        using C = CfgXR<R, N, true>;
         if (mode == FM_ADJ5)
           hipLaunchKernelGGL((k_fwd_xr_adj<R, N, true>), dim3(cdiv(G::KZ, C::L), G::K), dim3(C::NT), 0, s, in, a, tw);
         else
@@ -260,7 +260,7 @@ struct Launch {
   }
   // blocks of the adjoint forward-x pass (slots of the nu-gradient partials)
   static int adj_x_blocks() {
-    if constexpr (use_xr<N>()) return cdiv(G::KZ, CfgXR<R, N>::L) * G::K;
+    if constexpr (use_xr<N>()) return cdiv(G::KZ, CfgXR<R, N, true>::L) * G::K;
     else return cdiv(G::KZ, CfgXF<R, N, 0>::LINES / 6) * G::K;
   }
   static int diag(const cpx<R>* su, const cpx<R>* sb, double* part, int nblk, double ks, hipStream_t s) {

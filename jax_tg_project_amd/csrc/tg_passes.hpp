@@ -1710,11 +1710,14 @@ __global__ __launch_bounds__((CfgXF<R, N, V>::NT)) void k_fwd_x(const cpx<R>* __
 // viscous term and RK4 update run on the thread's own (kx, kz) points with
 // no LDS round trip.  LDS holds one field instead of three, so several
 // blocks share a CU and overlap their load, transform and update phases.
-template <typename R, int N>
+template <typename R, int N, bool ADJ = false>
 struct CfgXR {
   // power-of-two grids of N >= 128 only (use_xr); 4 points per thread at
-  // N = 128 (twice the wavefronts of 8 on the small grid), else 8
-  static constexpr int E = N <= 128 ? 4 : 8;
+  // N = 128 (twice the wavefronts of 8 on the small grid), else 8; the FP32
+  // RK4 stage at N = 256 takes 16 (one LDS exchange instead of two, three
+  // wavefronts per SIMD without spills; the adjoint's five fields and its
+  // epilogue spill at 16)
+  static constexpr int E = N <= 128 ? 4 : (sizeof(R) == 4 && N == 256 && !ADJ ? 16 : 8);
   // one 128-byte line of kz per row, halved in FP32 above 512 threads
   static constexpr int L = (sizeof(R) == 8 || 16 * N / E <= 512) ? 128 / int(2 * sizeof(R)) : 8;
   static constexpr int NT = L * N / E;
@@ -1756,7 +1759,7 @@ enum XRKind { XR_HYDRO = 0, XR_MHD = 1, XR_ADJ = 2, XR_ADJ5 = 3 };
 template <typename R, int N, int KIND = XR_HYDRO>
 struct XRBody {
   using G = Geo<N>;
-  using C = CfgXR<R, N>;
+  using C = CfgXR<R, N, KIND == XR_ADJ || KIND == XR_ADJ5>;
   static constexpr bool MHD = KIND == XR_MHD;
   static constexpr int SMEM = N * C::L + N;
   // fields per block: n (hydro); m or e (MHD: blockIdx.z = 0 updates u from
@@ -2016,7 +2019,7 @@ __attribute__((amdgpu_waves_per_eu(xr_waves_per_eu<R, MHD ? XR_MHD : XR_HYDRO>()
 // Forward x of one stage VJP with its adjoint epilogue on register-held
 // lines (grids of N >= 128; one 128-byte line of kz columns per block).
 template <typename R, int N, bool FOLDED>
-__global__ __launch_bounds__((CfgXR<R, N>::NT)) __attribute__((amdgpu_waves_per_eu(xr_waves_per_eu<R, XR_ADJ>())))
+__global__ __launch_bounds__((CfgXR<R, N, true>::NT)) __attribute__((amdgpu_waves_per_eu(xr_waves_per_eu<R, XR_ADJ>())))
 void k_fwd_xr_adj(const cpx<R>* __restrict__ in, StageArgs<R> a,
                                                                  const cpx<R>* __restrict__ tw) {
   using B = XRBody<R, N, FOLDED ? XR_ADJ5 : XR_ADJ>;
