@@ -303,3 +303,37 @@ def test_recompute_variants_change_gradient_by_round_off_only(dtype):
     assert p.get_recompute_variants() == best and len(seen) >= 4
     with pytest.raises(Exception):
         p.set_recompute_variants([3, 0, 0, 0, 0])
+
+
+def test_sweep_at_256_fd():
+    """The reverse sweep at 256^3 (8 points per lane in the register forward x,
+    the five-field forward y, the fused inverse x of q) against central
+    differences: 6 steps in segments of 2 (one recomputed), J = Z(T), three
+    seeded directions and nu, relative 1e-6."""
+    from jax_tg_project_amd import ensemble
+    N, nu, dt, n, seg = 256, 1.0 / 1600, 2e-3, 6, 2
+    p = _plan(N)
+    x = p.forward(torch.tensor(ensemble.perturbed_taylor_green(N, 2024), device="cuda"))
+    aws = p.adjoint_workspace(n, seg)
+    J = p.adjoint_forward(x, n, dt, nu, 1, seg, aws)
+    g, dnu = p.adjoint_backward(n, dt, nu, seg, aws)
+    del aws
+
+    def Jof(xx, nuv):
+        y = xx.clone()
+        p.rk4(y, nsteps=n, dt=dt, nu=nuv)
+        return p.diagnostics(y, divmax=False)["enstrophy"]
+
+    rng = np.random.default_rng(256)
+    eps = 1e-6
+    xn = torch.linalg.vector_norm(x).item()
+    for d in range(3):
+        v = torch.tensor(rng.standard_normal(x.shape) + 1j * rng.standard_normal(x.shape), device="cuda")
+        v *= xn / torch.linalg.vector_norm(v).item()
+        fd = (Jof(x + eps * v, nu) - Jof(x - eps * v, nu)) / (2 * eps)
+        an = float(torch.sum(torch.real(torch.conj(g) * v)))
+        assert abs(an - fd) <= 1e-6 * abs(fd), (d, an, fd)
+    h = 1e-5 * nu
+    fdn = (Jof(x, nu + h) - Jof(x, nu - h)) / (2 * h)
+    assert abs(dnu - fdn) <= 1e-6 * abs(fdn), (dnu, fdn)
+    assert abs(J - Jof(x, nu)) < 1e-14 * J
