@@ -44,6 +44,17 @@ constexpr int kzp() {
 // next pass while still cached) and smaller grids use ordinary accesses.
 template <int N>
 constexpr bool nt_mid() { return N >= 256; }
+// The exceptions are outputs that the next pass reads at once: the x pass's
+// output (5 fields, read by the inverse y), the hydro z pass's (3 fields,
+// read by the forward y) and, in a stage VJP, the fused inverse x of the
+// next cotangent (3 fields).  While such an output is not much larger
+// than the last-level cache it is stored with ordinary stores, so most of it
+// is still cached when the next pass reads it.  At 256^3 (FP64: 308 and 277
+// MB; FP32: 168 and 151 MB) this took FP64 from 402 to 422 steps/s and FP32
+// from 745 to 765.  Cached stores of the larger inverse-y and forward-y
+// outputs, or of every array, were measured slower, and so were cached
+// stores of the VJP's y-pass outputs (the recompute stream shares the cache).
+constexpr bool nt_store_for(long long bytes) { return bytes > (320LL << 20); }
 
 // Launch shapes.  Grids of N >= 128 compile NVAR tile variants per pass;
 // the plan picks one per pass at run time (see the autotuner); smaller grids
@@ -278,6 +289,9 @@ struct XInvBody {
                              const cpx<R>* __restrict__ tw, R ks, R alpha = R(1), int cpz = Geo<N>::KZ) {
     constexpr int L = C::L, E = C::E;
     constexpr int F = xmode_fields(MODE);
+    // the launch's whole output (both vectors of the MHD5 split)
+    constexpr bool NT_OUT = nt_store_for(
+        (long long)(MODE == XM_MHD5 ? 2 * F : F) * N * G::K * kzp<R, N>() * (long long)sizeof(cpx<R>));
     cpx<R>* buf = sm;
     cpx<R>* tws = sm + N * L;
     const int iz0 = bx * L, iy = by;
@@ -356,7 +370,7 @@ struct XInvBody {
       };
       auto st = [&](int n, int ll, cpx<R> v, int) {
         const int izw = iz0 + ll;
-        if (izw < G::KZ) gst<nt_mid<N>()>(&out[((size_t(f) * N + n) * G::K + iy) * kzp<R, N>() + izw], v);
+        if (izw < G::KZ) gst<NT_OUT>(&out[((size_t(f) * N + n) * G::K + iy) * kzp<R, N>() + izw], v);
       };
       fft_io<R, N, E, +1, 1, true>(buf, L, L, tws, ld, st);
     }
@@ -1105,12 +1119,12 @@ __device__ __forceinline__ void zw_split_pair(const cpx<R>* x, cpx<R>* lb, int t
 
 // Store the kept kz of the two real fields packed in x into fields FA (real
 // part) and FB of dst.
-template <typename R, int N, class C, int FA, int FB>
+template <typename R, int N, class C, int FA, int FB, bool NT = nt_mid<N>()>
 __device__ __forceinline__ void zw_store_pair(const cpx<R>* x, cpx<R>* lb, cpx<R>* __restrict__ dst, size_t NNKZ,
                                               int t) {
   zw_split_pair<R, N, C>(x, lb, t, [&](int, int kz, cpx<R> a, cpx<R> b) {
-    gst<nt_mid<N>()>(&dst[FA * NNKZ + kz], a);
-    gst<nt_mid<N>()>(&dst[FB * NNKZ + kz], b);
+    gst<NT>(&dst[FA * NNKZ + kz], a);
+    gst<NT>(&dst[FB * NNKZ + kz], b);
   });
 }
 
@@ -1127,6 +1141,7 @@ struct ZWBody {
     constexpr int KZP = PITCH;
     constexpr int R1 = FirstRadix<IP>::value, NB1 = E / R1, S1 = N / R1;
     const size_t NNKZ = size_t(N) * N * KZP;
+    constexpr bool NT_OUT = nt_store_for(3LL * N * N * KZP * (long long)sizeof(cpx<R>));
     cpx<R>* tsi = sm;
     cpx<R>* tsf = C::SAME ? sm : sm + C::TWI;
     PlanTw<N, IP>::fill(tsi, tw, threadIdx.x, C::NT);
@@ -1177,11 +1192,11 @@ struct ZWBody {
       for (int m = 0; m < R1; ++m) {
         if (j + m * S1 < KZ) {
           const int k = t * NB1 + j + m * S1;
-          if (k < KZ) gst<nt_mid<N>()>(&dst[2 * NNKZ + k], c[j * R1 + m]);
+          if (k < KZ) gst<NT_OUT>(&dst[2 * NNKZ + k], c[j * R1 + m]);
         }
       }
     // ---- n_x, n_y from the pair (k, N - k) of the packed transform
-    zw_store_pair<R, N, C, 0, 1>(a, lb, dst, NNKZ, t);
+    zw_store_pair<R, N, C, 0, 1, NT_OUT>(a, lb, dst, NNKZ, t);
   }
 };
 
@@ -1864,7 +1879,9 @@ struct XRBody {
           auto ld = [&](int, int, int slot) -> cpx<R> { return qn[f][slot]; };
           auto st = [&](int n, int ll, cpx<R> v, int) {
             const int izw = iz0 + ll;
-            if (izw < G::KZ) gst<nt_mid<N>()>(&a.qout[((size_t(f) * N + n) * G::K + iy) * kzp<R, N>() + izw], v);
+            if (izw < G::KZ)
+              gst<nt_store_for(3LL * N * G::K * kzp<R, N>() * (long long)sizeof(cpx<R>))>(
+                  &a.qout[((size_t(f) * N + n) * G::K + iy) * kzp<R, N>() + izw], v);
           };
           fft_io_plan<R, N, E, +1, 1, true, IP>(buf, L, L, tws, ld, st);
         }
