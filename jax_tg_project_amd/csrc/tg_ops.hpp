@@ -116,21 +116,13 @@ struct Launch {
     });
   }
   // Inverse y of `nvec` vectors handed over as five fields each (HYDRO5 /
-  // MHD5 x pass), writing (v, i k x v) per vector.  slot: `out` is an
-  // adjoint stage slot (see below).
-  static int y_curl(int var, int nvec, const cpx<R>* in, cpx<R>* out, const cpx<R>* tw, R ks, hipStream_t s,
-                    bool slot = false) {
+  // MHD5 x pass), writing (v, i k x v) per vector.
+  static int y_curl(int var, int nvec, const cpx<R>* in, cpx<R>* out, const cpx<R>* tw, R ks, hipStream_t s) {
     return with_variant<N>(var, [&](auto vc) {
       constexpr int V = decltype(vc)::value;
       using C = CfgXY<R, N, V>;
       const int units = cdiv(G::KZ, C::L) * N * nvec;
-      // an adjoint stage slot is read again only by that stage's VJP, much
-      // later: stored past the cache (128^3 sqrt(n) adjoint 2.955x -> 2.94x)
-      if (slot)
-        hipLaunchKernelGGL((k_y_curl<R, N, V, true>), dim3(48 * cdiv(units, 8)), dim3(C::NT), 0, s, in, out, tw, ks,
-                           units);
-      else
-        hipLaunchKernelGGL((k_y_curl<R, N, V>), dim3(48 * cdiv(units, 8)), dim3(C::NT), 0, s, in, out, tw, ks, units);
+      hipLaunchKernelGGL((k_y_curl<R, N, V>), dim3(48 * cdiv(units, 8)), dim3(C::NT), 0, s, in, out, tw, ks, units);
       TG_LAUNCHED();
       return TG_OK;
     });
@@ -326,7 +318,7 @@ int rhs_passes(const tg_plan* p, const Ptrs<R>& w, const cpx<R>* us, const cpx<R
   TG_TIMED(TG_PASS_INV_X, Lc::inv_x(mhd ? XM_MHD5 : XM_HYDRO5, v[TG_PASS_INV_X], us, bs, w.A, tw, R(p->ks), s, R(1),
                                     cpz));
   cpx<R>* bm = bslot ? bslot : w.B;
-  TG_TIMED(TG_PASS_INV_Y, Lc::y_curl(v[TG_PASS_INV_Y], mhd ? 2 : 1, w.A, bm, tw, R(p->ks), s, bslot != nullptr));
+  TG_TIMED(TG_PASS_INV_Y, Lc::y_curl(v[TG_PASS_INV_Y], mhd ? 2 : 1, w.A, bm, tw, R(p->ks), s));
   TG_TIMED(TG_PASS_Z, Lc::z(mhd ? ZM_MHD : ZM_HYDRO, v[TG_PASS_Z], bm, w.A, nullptr, nullptr, tw, lor, s));
   TG_TIMED(TG_PASS_FWD_Y, Lc::y(-1, v[TG_PASS_FWD_Y], fo, w.A, w.B, tw, s));
   TG_TIMED(TG_PASS_FWD_X, Lc::fwd_x(fmode, v[TG_PASS_FWD_X], w.B, a, tw, s));

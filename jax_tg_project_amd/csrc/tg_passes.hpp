@@ -530,7 +530,7 @@ __global__ __launch_bounds__((CfgXY<R, N, V>::NT)) void k_y_adj(const cpx<R>* __
 // six blocks of that tile, with ordinary loads so that the copies the other
 // blocks fetched serve them from L2 (the launcher places the six blocks on
 // one XCD, one after another), and D_y, D_z once, streamed.
-template <typename R, int N, int V, bool NTO = nt_mid<N>()>
+template <typename R, int N, int V>
 struct YCurlBody {
   using G = Geo<N>;
   using C = CfgXY<R, N, V>;
@@ -570,7 +570,7 @@ struct YCurlBody {
     };
     auto st = [&](int n, int ll, cpx<R> v, int) {
       const int iz = iz0 + ll;
-      if (iz < G::KZ) gst<NTO>(&b[F * BF + size_t(n) * KZP + iz], v);
+      if (iz < G::KZ) gst<NT>(&b[F * BF + size_t(n) * KZP + iz], v);
     };
     fft_io<R, N, E, +1, 1, true>(sm, L, L, tws, ld, st);
   }
@@ -600,11 +600,11 @@ struct YCurlBody {
 // vectors: workgroups go to the 8 XCDs round robin (b mod 8), so the six
 // fields of tile u take the consecutive slots 6 j .. 6 j + 5 of XCD u mod 8
 // (j = u / 8).  Blocks past the last tile return at once.
-template <typename R, int N, int V, bool NTO = nt_mid<N>()>
+template <typename R, int N, int V>
 __global__ __launch_bounds__((CfgXY<R, N, V>::NT)) void k_y_curl(const cpx<R>* __restrict__ in,
                                                                cpx<R>* __restrict__ out,
                                                                const cpx<R>* __restrict__ tw, R ks, int units) {
-  using B = YCurlBody<R, N, V, NTO>;
+  using B = YCurlBody<R, N, V>;
   constexpr int T = (Geo<N>::KZ + CfgXY<R, N, V>::L - 1) / CfgXY<R, N, V>::L;
   __shared__ cpx<R> sm[B::SMEM];
   const int b = blockIdx.x, xcd = b & 7, j = b >> 3;
