@@ -37,6 +37,28 @@ struct EnsInst {
   const cpx<R>* tw;
 };
 
+// The descriptor as the kernels read it: the same layout, with every pointer
+// in the global address space.  A pointer loaded from memory through the
+// generic pointers of EnsInst is generic to the compiler, which then accesses
+// it with flat instructions; those count in both the vector-memory and the
+// LDS wait counters, so every LDS wait of a transform would also wait for the
+// wave's outstanding loads and stores.  Descriptors only point into device
+// memory.
+#define TG_GLOBAL_AS __attribute__((address_space(1)))
+template <typename R>
+struct EnsInstG {
+  int n;
+  R nu, ks;
+  TG_GLOBAL_AS cpx<R>* s0;
+  TG_GLOBAL_AS cpx<R>* u1;
+  TG_GLOBAL_AS cpx<R>* acc;
+  TG_GLOBAL_AS cpx<R>* A;
+  TG_GLOBAL_AS cpx<R>* B;
+  TG_GLOBAL_AS const cpx<R>* tw;
+};
+static_assert(sizeof(EnsInstG<float>) == sizeof(EnsInst<float>) && sizeof(EnsInstG<double>) == sizeof(EnsInst<double>),
+              "one descriptor layout");
+
 inline __host__ __device__ constexpr int cdiv_c(int a, int b) { return (a + b - 1) / b; }
 
 // Blocks of one instance's grid for each pass.
@@ -66,12 +88,18 @@ struct EnsGrid {
 };
 
 template <typename R, int N, int PASS>
-__device__ __forceinline__ void ens_run(cpx<R>* sm, int lb, const EnsInst<R>& I, R dt, int stage) {
+__device__ __forceinline__ void ens_run(cpx<R>* sm, int lb, const EnsInstG<R>& I, R dt, int stage) {
   using Gd = EnsGrid<R, N>;
   constexpr int KZ = Geo<N>::KZ;
-  const cpx<R>* us = stage == 0 ? I.s0 : I.u1;
+  // explicit casts: the host pass has no implicit global -> generic conversion
+  cpx<R>* const s0 = (cpx<R>*)I.s0;
+  cpx<R>* const u1 = (cpx<R>*)I.u1;
+  cpx<R>* const A = (cpx<R>*)I.A;
+  cpx<R>* const B = (cpx<R>*)I.B;
+  const cpx<R>* const tw = (const cpx<R>*)I.tw;
+  const cpx<R>* us = stage == 0 ? s0 : u1;
   if constexpr (PASS == EP_INV_X) {
-    EnsInvX<R, N>::run(sm, lb, us, I.A, I.tw, I.ks);
+    EnsInvX<R, N>::run(sm, lb, us, A, tw, I.ks);
   } else if constexpr (PASS == EP_INV_Y) {
     int g, f;
     if constexpr (Gd::XCD_GROUPED) {
@@ -82,27 +110,27 @@ __device__ __forceinline__ void ens_run(cpx<R>* sm, int lb, const EnsInst<R>& I,
       g = lb % Gd::GROUPS;
       f = lb / Gd::GROUPS;
     }
-    EnsInvY<R, N>::run(sm, g, f, I.A, I.B, I.tw, I.ks);
+    EnsInvY<R, N>::run(sm, g, f, A, B, tw, I.ks);
   } else if constexpr (PASS == EP_FWD_Y) {
-    EnsFwdY<R, N>::run(sm, lb % Gd::GROUPS, lb / Gd::GROUPS, I.A, I.B, I.tw);
+    EnsFwdY<R, N>::run(sm, lb % Gd::GROUPS, lb / Gd::GROUPS, A, B, tw);
   } else if constexpr (PASS == EP_Z) {
     if constexpr (Gd::ZE > 0) {
-      ZWBody<R, N, Gd::ZE, Gd::ZWPB, KZ>::run(sm, lb, I.B, I.A, I.tw);
+      ZWBody<R, N, Gd::ZE, Gd::ZWPB, KZ>::run(sm, lb, B, A, tw);
     } else {
       constexpr int gx = N / Gd::LZ;
-      ZBody<R, N, ZM_HYDRO, 0, KZ>::run(sm, lb % gx, lb / gx, I.B, I.A, nullptr, nullptr, I.tw, R(0));
+      ZBody<R, N, ZM_HYDRO, 0, KZ>::run(sm, lb % gx, lb / gx, B, A, nullptr, nullptr, tw, R(0));
     }
   } else {
     EnsStage<R> a;
     a.us = us;
-    a.u0 = I.s0;
-    a.unext = stage == 3 ? I.s0 : I.u1;
-    a.acc = I.acc;
+    a.u0 = s0;
+    a.unext = stage == 3 ? s0 : u1;
+    a.acc = (cpx<R>*)I.acc;
     a.dt = dt;
     a.nu = I.nu;
     a.ks = I.ks;
     a.stage = stage;
-    EnsFwdX<R, N>::run(sm, lb, I.B, a, I.tw);
+    EnsFwdX<R, N>::run(sm, lb, B, a, tw);
   }
 }
 
@@ -148,13 +176,14 @@ __attribute__((amdgpu_waves_per_eu(EnsBlock<R, N, PASS>::WAVES))) void kb_pass(c
     if (prefix[mid] <= b) lo = mid;
     else hi = mid;
   }
-  ens_run<R, N, PASS>(sm, b - prefix[lo], inst[lo], dt, stage);
+  ens_run<R, N, PASS>(sm, b - prefix[lo], reinterpret_cast<const EnsInstG<R>*>(inst)[lo], dt, stage);
 }
 
 // Sizes whose batched x bodies (256 threads) and z bodies (one wavefront)
 // share their block shapes, so one launch per pass can cover instances of
 // all of them: a block finds its instance as in kb_pass and branches (block-
-// uniformly) to that instance's size body.  The y bodies differ in threads
+// uniformly) to that instance's size body.  rev_n > 0 (the launch's block
+// count) runs the blocks in reverse order.  The y bodies differ in threads
 // (PPB planes x KZ x N/E); the mixed y launch takes the largest, and the
 // threads a smaller body does not use stay inactive through its transforms
 // while still reaching every barrier.
@@ -189,9 +218,9 @@ template <typename R, int PASS>
 __global__ __launch_bounds__((MixBlock<R, PASS>::NT))
 __attribute__((amdgpu_waves_per_eu(MixBlock<R, PASS>::WAVES))) void kb_mixed(const EnsInst<R>* __restrict__ inst,
                                                                       const int* __restrict__ prefix, int ninst,
-                                                                      R dt, int stage) {
+                                                                      R dt, int stage, int rev_n) {
   __shared__ cpx<R> sm[MixBlock<R, PASS>::SMEM];
-  const int b = blockIdx.x;
+  const int b = rev_n ? rev_n - 1 - int(blockIdx.x) : int(blockIdx.x);
   int lo = 0, hi = ninst;  // prefix[lo] <= b < prefix[hi]
   while (hi - lo > 1) {
     const int mid = (lo + hi) >> 1;
@@ -199,11 +228,12 @@ __attribute__((amdgpu_waves_per_eu(MixBlock<R, PASS>::WAVES))) void kb_mixed(con
     else hi = mid;
   }
   const int lb = b - prefix[lo];
-  switch (inst[lo].n) {
-    case 96: ens_run<R, 96, PASS>(sm, lb, inst[lo], dt, stage); break;
-    case 64: ens_run<R, 64, PASS>(sm, lb, inst[lo], dt, stage); break;
-    case 48: ens_run<R, 48, PASS>(sm, lb, inst[lo], dt, stage); break;
-    default: ens_run<R, 32, PASS>(sm, lb, inst[lo], dt, stage); break;
+  const EnsInstG<R>& I = reinterpret_cast<const EnsInstG<R>*>(inst)[lo];
+  switch (I.n) {
+    case 96: ens_run<R, 96, PASS>(sm, lb, I, dt, stage); break;
+    case 64: ens_run<R, 64, PASS>(sm, lb, I, dt, stage); break;
+    case 48: ens_run<R, 48, PASS>(sm, lb, I, dt, stage); break;
+    default: ens_run<R, 32, PASS>(sm, lb, I, dt, stage); break;
   }
 }
 
@@ -256,8 +286,13 @@ struct EnsMixTable {
 template <typename R, int PASS>
 int mix_launch(const EnsInst<R>* inst, const int* dpre, const int* hpre, int cnt, int st, double dt, hipStream_t s) {
   const int nblk = hpre[PASS * (cnt + 1) + cnt];
+  // Every other launch of a step (20 per step, so also across steps) runs its
+  // blocks in reverse order: a pass then first reads what the previous pass
+  // wrote last, which is still in the last-level cache when the previous
+  // pass's output exceeds it.
+  const bool rev = (st * EP_COUNT + PASS) & 1;
   hipLaunchKernelGGL((kb_mixed<R, PASS>), dim3(nblk), dim3(MixBlock<R, PASS>::NT), 0, s, inst,
-                     dpre + PASS * (cnt + 1), cnt, R(dt), st);
+                     dpre + PASS * (cnt + 1), cnt, R(dt), st, rev ? nblk : 0);
   const hipError_t e = hipGetLastError();
   return e == hipSuccess ? TG_OK : fail(TG_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
 }
