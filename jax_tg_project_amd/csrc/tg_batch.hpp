@@ -59,6 +59,15 @@ struct EnsInstG {
 static_assert(sizeof(EnsInstG<float>) == sizeof(EnsInst<float>) && sizeof(EnsInstG<double>) == sizeof(EnsInst<double>),
               "one descriptor layout");
 
+// Descriptor i, read as its global-pointer view (a byte copy: the two
+// structs share the layout but not the type).
+template <typename R>
+__device__ __forceinline__ EnsInstG<R> load_inst(const EnsInst<R>* inst, int i) {
+  EnsInstG<R> g;
+  __builtin_memcpy(&g, inst + i, sizeof(g));
+  return g;
+}
+
 inline __host__ __device__ constexpr int cdiv_c(int a, int b) { return (a + b - 1) / b; }
 
 // Blocks of one instance's grid for each pass.
@@ -176,7 +185,7 @@ __attribute__((amdgpu_waves_per_eu(EnsBlock<R, N, PASS>::WAVES))) void kb_pass(c
     if (prefix[mid] <= b) lo = mid;
     else hi = mid;
   }
-  ens_run<R, N, PASS>(sm, b - prefix[lo], reinterpret_cast<const EnsInstG<R>*>(inst)[lo], dt, stage);
+  ens_run<R, N, PASS>(sm, b - prefix[lo], load_inst(inst, lo), dt, stage);
 }
 
 // Sizes whose batched x bodies (256 threads) and z bodies (one wavefront)
@@ -228,7 +237,7 @@ __attribute__((amdgpu_waves_per_eu(MixBlock<R, PASS>::WAVES))) void kb_mixed(con
     else hi = mid;
   }
   const int lb = b - prefix[lo];
-  const EnsInstG<R>& I = reinterpret_cast<const EnsInstG<R>*>(inst)[lo];
+  const EnsInstG<R> I = load_inst(inst, lo);
   switch (I.n) {
     case 96: ens_run<R, 96, PASS>(sm, lb, I, dt, stage); break;
     case 64: ens_run<R, 64, PASS>(sm, lb, I, dt, stage); break;
