@@ -932,6 +932,39 @@ struct ZWPlan {
 };
 template <> struct ZWPlan<128, 8> { using type = Radices<8, 4, 4>; };
 
+// LDS layout of the register z bodies' line buffers: one pad slot every
+// 2^PS points (pidx) and a line pitch of LPL points.  The default keeps
+// consecutive lines 32 banks apart.  In FP32, 256 bytes of lanes (32 lanes:
+// the share of one LDS cycle at 64 banks of 4 bytes) hold 4 to 16 lines on
+// the grids below 256, and the default layout puts those lines on the same
+// banks: 2 to 4 times the conflict-free cycles, and 5.4 conflict cycles
+// per LDS instruction measured in the ensemble's z launch.  Those shapes
+// take the PS and LPL that a model of the banks gives the fewest cycles
+// over every put and get of the line transforms and the Hermitian split
+// (the smallest LPL on ties).  The FP64 shapes keep the default: the
+// model's choice for 128^3 at 8 points per lane ran slower.
+template <typename R, int N, int E>
+struct ZwLds {
+  static constexpr int PS = ilog2(E);
+  static constexpr int USED = N + (N >> PS);
+  static constexpr int LPL = sizeof(R) == 8 ? (USED + 7) / 16 * 16 + 8 : (USED + 15) / 32 * 32 + 16;
+};
+#define TG_ZW_LDS(R_, N_, E_, PS_, LPL_)                 \
+  template <>                                           \
+  struct ZwLds<R_, N_, E_> {                            \
+    static constexpr int PS = PS_, LPL = LPL_;          \
+    static constexpr int USED = N_ + (N_ >> PS_);       \
+    static_assert(LPL_ >= USED, "line pitch below the padded line"); \
+  };
+TG_ZW_LDS(float, 32, 8, 3, 36)
+TG_ZW_LDS(float, 32, 16, 4, 34)
+TG_ZW_LDS(float, 48, 12, 2, 60)
+TG_ZW_LDS(float, 64, 8, 3, 72)
+TG_ZW_LDS(float, 64, 16, 3, 76)
+TG_ZW_LDS(float, 96, 12, 1, 152)
+TG_ZW_LDS(float, 128, 16, 4, 136)
+#undef TG_ZW_LDS
+
 template <typename R, int N, int E_, int WPB>
 struct CfgZW {
   static constexpr int E = E_;
@@ -940,11 +973,9 @@ struct CfgZW {
   static_assert(N % E == 0 && 64 % TPL == 0, "a wavefront holds whole lines");
   static constexpr int LINES = WPB * LW;     // lines per block
   static constexpr int NT = 64 * WPB;
-  static constexpr int PS = ilog2(E);
-  static constexpr int USED = N + (N >> PS);
-  // line pitch: consecutive lines start 32 banks apart (8 mod 16 slots of
-  // 16 bytes for FP64, 16 mod 32 slots of 8 bytes for FP32)
-  static constexpr int LPL = sizeof(R) == 8 ? (USED + 7) / 16 * 16 + 8 : (USED + 15) / 32 * 32 + 16;
+  static constexpr int PS = ZwLds<R, N, E>::PS;
+  static constexpr int USED = ZwLds<R, N, E>::USED;
+  static constexpr int LPL = ZwLds<R, N, E>::LPL;
   using IP = typename ZWPlan<N, E>::type;
   using FP = ReversePlan<IP>;
   static constexpr bool SAME = std::is_same<IP, FP>::value;
