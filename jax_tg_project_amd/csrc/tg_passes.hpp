@@ -941,8 +941,10 @@ template <> struct ZWPlan<128, 8> { using type = Radices<8, 4, 4>; };
 // per LDS instruction measured in the ensemble's z launch.  Those shapes
 // take the PS and LPL that a model of the banks gives the fewest cycles
 // over every put and get of the line transforms and the Hermitian split
-// (the smallest LPL on ties).  The FP64 shapes keep the default: the
-// model's choice for 128^3 at 8 points per lane ran slower.
+// (the smallest LPL on ties).  In FP64 the model's choices ran slower; the
+// 128^3 shape at 8 points per lane (the MHD and adjoint z passes) takes the
+// best of a few measured layouts (unpadded pitch: adjoint -0.7%), the
+// others keep the default.
 template <typename R, int N, int E>
 struct ZwLds {
   static constexpr int PS = ilog2(E);
@@ -963,6 +965,7 @@ TG_ZW_LDS(float, 64, 8, 3, 72)
 TG_ZW_LDS(float, 64, 16, 3, 76)
 TG_ZW_LDS(float, 96, 12, 1, 152)
 TG_ZW_LDS(float, 128, 16, 4, 136)
+TG_ZW_LDS(double, 128, 8, 3, 144)
 #undef TG_ZW_LDS
 
 template <typename R, int N, int E_, int WPB>
