@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstring>
 #include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "tg_common.hpp"
@@ -460,12 +461,18 @@ struct Ops {
   // 4 seg stage states | RHS workspace.
   // On grids with the register adjoint z pass, each stage buffer has a
   // matching set of 4 seg slots for the stages' inverse-y output (u, w).
+  // With more than two segments a third stage buffer (4 seg stage states,
+  // then its slots) holds the third-last segment, which the forward sweep
+  // keeps as well.
   struct AdjLayout {
     cpx<R>* cur; cpx<R>* lamp; cpx<R>* lacc; cpx<R>* abuf; cpx<R>* ck; cpx<R>* st; double* nupart;
     cpx<R>* cur2; cpx<R>* st2; void* ws2;
     cpx<R>* slots; cpx<R>* slots2;
+    cpx<R>* st3; cpx<R>* slots3;
     size_t vstride, sstride; int nck, nblk;
   };
+  // Segments the forward sweep keeps (their stage states): the last three.
+  static constexpr int kKeptSegments = 3;
   // one padded 3-component vector
   static size_t vec_bytes(const tg_plan* p) { return align256(size_t(3) * p->K * p->K * kzp<R, N>() * sizeof(cpx<R>)); }
   static size_t slot_bytes(const tg_plan* p) { return align256(size_t(6) * N * N * kzp<R, N>() * sizeof(cpx<R>)); }
@@ -490,6 +497,9 @@ struct Ops {
     const size_t sb = slot_bytes(p);
     L.slots = adj_reg_z<N>() ? reinterpret_cast<cpx<R>*>(third) : nullptr;
     L.slots2 = (adj_reg_z<N>() && L.nck > 1) ? reinterpret_cast<cpx<R>*>(third + 4 * size_t(seg) * sb) : nullptr;
+    char* fourth = third + (adj_reg_z<N>() ? size_t(L.nck > 1 ? 2 : 1) * 4 * size_t(seg) * sb : 0);
+    L.st3 = L.nck > 2 ? reinterpret_cast<cpx<R>*>(fourth) : nullptr;
+    L.slots3 = (adj_reg_z<N>() && L.nck > 2) ? reinterpret_cast<cpx<R>*>(fourth + 4 * size_t(seg) * vb) : nullptr;
     L.sstride = sb / sizeof(cpx<R>);
     L.vstride = vb / sizeof(cpx<R>);
     L.nblk = Launch<R, N>::adj_x_blocks();
@@ -535,10 +545,20 @@ struct Ops {
     }
     return TG_OK;
   }
+  // Stage buffer (states, slots) of segment c in the reverse sweep: the last
+  // segment in the first buffer, the second-last in the second, and from the
+  // third-last down the third and the first in turn (segment c - 1 is
+  // recomputed into the buffer segment c + 1 used, or into the first for
+  // the fourth-last segment, while segment c is reversed).
+  static std::pair<cpx<R>*, cpx<R>*> stage_buffer(const AdjLayout& L, int c) {
+    if (c == L.nck - 1) return {L.st, L.slots};
+    if (c == L.nck - 2) return {L.st2, L.slots2};
+    return (L.nck - 3 - c) % 2 == 0 ? std::make_pair(L.st3, L.slots3) : std::make_pair(L.st, L.slots);
+  }
   // Forward sweep: nsteps RK4 steps from x0 with a checkpoint every `seg`
   // steps, J = E(T) (objective 0) or Z(T) (1), and the terminal cotangent.
-  // The stage states (and kept inverse-y outputs) of the last segment go to
-  // the first stage buffer and those of the segment before it to the second.
+  // The stage states (and kept inverse-y outputs) of the last three segments
+  // go to their stage buffers (stage_buffer).
   static int adj_forward(const tg_plan* p, const cpx<R>* x0, int nsteps, double dt, double nu, int objective,
                          int seg, double* J, void* ws, void* aws, hipStream_t s) {
     AdjLayout L = adj_layout(p, nsteps, seg, aws);
@@ -549,12 +569,12 @@ struct Ops {
     for (int c = 0; c < L.nck; ++c) {
       TG_HIP(hipMemcpyAsync(L.ck + c * L.vstride, L.cur, bytes, hipMemcpyDeviceToDevice, s));
       const int len = seg_len(nsteps, seg, c);
-      if (c < L.nck - 2) TG_RET(rk4_padded(p, L.cur, nullptr, len, dt, nu, 0.0, -1.0, ws, s));
-      else if (c == L.nck - 2)  // kept in the second stage buffer: the reverse sweep never recomputes it
-        TG_RET(steps_keeping_stages(p, L.cur, L.st2, L.vstride, len, dt, nu, ws, s, L.slots2, L.sstride));
-      else
-        TG_RET(steps_keeping_stages(p, L.cur, L.st, L.vstride, len, dt, nu, ws, s, L.slots,
-                                    L.sstride));  // the reverse sweep starts here
+      if (c < L.nck - kKeptSegments) {
+        TG_RET(rk4_padded(p, L.cur, nullptr, len, dt, nu, 0.0, -1.0, ws, s));
+      } else {  // kept: the reverse sweep never recomputes it
+        const auto b = stage_buffer(L, c);
+        TG_RET(steps_keeping_stages(p, L.cur, b.first, L.vstride, len, dt, nu, ws, s, b.second, L.sstride));
+      }
     }
     // diagnostics read the dense layout: a copy of the final state in abuf
     TG_HIP(hipMemcpy2DAsync(L.abuf, KZ * e, L.cur, KZP * e, KZ * e, rows, hipMemcpyDeviceToDevice, s));
@@ -582,11 +602,10 @@ struct Ops {
     const R h = R(dt);
     const R cl[4] = {h / R(6), h / R(3), h / R(3), h / R(6)};  // weight of lamp in a_s (s = 1..4)
     const R cm[4] = {R(0), h / R(2), h / R(2), h};              // weight of mu_{s+1} in a_s
-    // The stages of the last segment are kept by the forward sweep in buffer
-    // 0, those of the segment before it in buffer 1.
-    // Segment c - 1 is recomputed on the private stream into the other
-    // buffer while segment c is reversed on `s`; events order the buffer
-    // reuse both ways.
+    // The stages of the last three segments are kept by the forward sweep
+    // (stage_buffer).  Segment c - 1 is recomputed on the private stream
+    // into its buffer while segment c is reversed on `s`; events order the
+    // buffer reuse both ways.
     hipStream_t s2 = nullptr;
     tg_plan* mp = const_cast<tg_plan*>(p);
     auto ev_rec = [&](int c) { return mp->adjev[1 + 2 * size_t(c)]; };  // recompute of segment c done
@@ -611,18 +630,20 @@ struct Ops {
     bool q_in_A = false;
     for (int c = L.nck - 1; c >= 0; --c) {
       const int len = seg_len(nsteps, seg, c);
-      const bool even = (L.nck - 1 - c) % 2 == 0;
-      cpx<R>* stc = even ? L.st : L.st2;
-      cpx<R>* slc = even ? L.slots : L.slots2;
-      // segment c recomputed (the last two were kept by the forward sweep)
-      if (c < L.nck - 2) TG_HIP(hipStreamWaitEvent(s, ev_rec(c), 0));
-      // recompute segment c - 1 into the buffer segment c + 1 used; its steps
-      // are enqueued one per reversed step of segment c, so the host feeds
-      // both streams together instead of queueing the whole recompute first
-      const bool rec = c > 0 && c - 1 != L.nck - 2;
-      const bool evn = (L.nck - c) % 2 == 0;
-      cpx<R>* stn = evn ? L.st : L.st2;
-      cpx<R>* sln = evn ? L.slots : L.slots2;
+      const auto bc = stage_buffer(L, c);
+      cpx<R>* stc = bc.first;
+      cpx<R>* slc = bc.second;
+      // segment c recomputed (the last three were kept by the forward sweep)
+      if (c < L.nck - kKeptSegments) TG_HIP(hipStreamWaitEvent(s, ev_rec(c), 0));
+      // recompute segment c - 1 into its buffer (free: its last user was
+      // reversed earlier); its steps are enqueued one per reversed step of
+      // segment c, so the host feeds both streams together instead of
+      // queueing the whole recompute first
+      const bool rec = c > 0 && c - 1 < L.nck - kKeptSegments;
+      const auto bn = rec ? stage_buffer(L, c - 1) : std::make_pair(static_cast<cpx<R>*>(nullptr),
+                                                                     static_cast<cpx<R>*>(nullptr));
+      cpx<R>* stn = bn.first;
+      cpx<R>* sln = bn.second;
       const int lenn = rec ? seg_len(nsteps, seg, c - 1) : 0;
       int jr = 0;
       if (rec) {
@@ -657,7 +678,7 @@ struct Ops {
         TG_RET(keep_step(p, L.cur2, stn, L.vstride, jr, lenn, dt, nu, L.ws2, s2, sln, L.sstride, rv));
       if (rec) TG_HIP(hipEventRecord(ev_rec(c - 1), s2));
       if (c > 1) {
-        // the recompute of segment c - 2 reuses this buffer: after this reverse
+        // a later recompute reuses this buffer: after this reverse
         TG_HIP(hipEventRecord(ev_rev(c), s));
         TG_HIP(hipStreamWaitEvent(s2, ev_rev(c), 0));
       }

@@ -326,10 +326,11 @@ int tg_adjoint_workspace(tg_plan* p, int nsteps, int seg, long long* bytes) {
   const size_t nck = (nsteps + seg - 1) / seg;
   size_t b = (4 + nck + 4 * size_t(seg)) * vb + align256(size_t(p->K) * p->KZ * 8 * sizeof(double));
   if (nck > 1) b += (1 + 4 * size_t(seg)) * vb + align256(p->wsBytes);  // overlapped recompute
-  if (adj_reg_z_rt(p->n)) {  // kept inverse-y output (u, w) per stage, per stage buffer
-    const size_t sb = align256(6 * size_t(p->n) * p->n * kzpad * p->cb);
+  const size_t sb = align256(6 * size_t(p->n) * p->n * kzpad * p->cb);
+  if (adj_reg_z_rt(p->n))  // kept inverse-y output (u, w) per stage, per stage buffer
     b += (nck > 1 ? 2 : 1) * 4 * size_t(seg) * sb;
-  }
+  if (nck > 2)  // third stage buffer (the third-last segment is kept too)
+    b += 4 * size_t(seg) * (vb + (adj_reg_z_rt(p->n) ? sb : 0));
   *bytes = (long long)b;
   return TG_OK;
 }
