@@ -70,7 +70,7 @@ def test_vjp_matches_torch_autograd(N):
 
 @pytest.mark.parametrize("kind", ["energy", "enstrophy"])
 def test_gradient_matches_torch_autograd(kind):
-    N, nu, dt, n, seg = 16, 0.01, 0.01, 12, 4
+    N, nu, dt, n, seg = 16, 0.01, 0.01, 12, 3  # four segments: the first recomputed
     x = _state(N, 6)
     c = torch.tensor(x, requires_grad=True)
     nut = torch.tensor(nu, dtype=torch.float64, requires_grad=True)
@@ -90,9 +90,10 @@ def test_gradient_matches_torch_autograd(kind):
 def test_gradient_radix12_grid_matches_torch_autograd():
     """N = 48 (radix-12 plans, not a power of two): the VJPs transform the
     state and the cotangent (no kept stage fields), the workspace vectors
-    use the line-padded pitch (KZ = 17 -> 24), and seg = 2 over 6 steps
-    recomputes two segments on the overlapping stream."""
-    N, nu, dt, n, seg = 48, 0.02, 0.01, 6, 2
+    use the line-padded pitch (KZ = 17 -> 24), and seg = 1 over 6 steps
+    recomputes three segments on the overlapping stream (the forward sweep
+    keeps the last three)."""
+    N, nu, dt, n, seg = 48, 0.02, 0.01, 6, 1
     x = _state(N, 11)
     c = torch.tensor(x, requires_grad=True)
     nut = torch.tensor(nu, dtype=torch.float64, requires_grad=True)
@@ -274,10 +275,10 @@ def test_recompute_variants_change_gradient_by_round_off_only(dtype):
     """The reverse sweep's recompute may run its own tile variants (chosen by
     autotune.tune_recompute for running beside the reverse sweep): the
     gradient then differs from the plan's-own-variants gradient by round-off
-    only.  N = 128, 11 steps in segments of 3 (a short first segment of 2,
+    only.  N = 128, 17 steps in segments of 3 (a short first segment of 2,
     three recomputed segments), every uniform recompute set and the tuned one."""
     from jax_tg_project_amd import autotune, ensemble
-    N, nu, dt, n, seg = 128, 1.0 / 400, 5e-3, 11, 3
+    N, nu, dt, n, seg = 128, 1.0 / 400, 5e-3, 17, 3
     p = _plan(N, dtype)
     x = p.forward(torch.tensor(ensemble.perturbed_taylor_green(N, 2024), dtype=dtype, device="cuda"))
     aws = p.adjoint_workspace(n, seg)
@@ -308,10 +309,10 @@ def test_recompute_variants_change_gradient_by_round_off_only(dtype):
 def test_sweep_at_256_fd():
     """The reverse sweep at 256^3 (8 points per lane in the register forward x,
     the five-field forward y, the fused inverse x of q) against central
-    differences: 6 steps in segments of 2 (one recomputed), J = Z(T), three
+    differences: 8 steps in segments of 2 (one recomputed), J = Z(T), three
     seeded directions and nu, relative 1e-6."""
     from jax_tg_project_amd import ensemble
-    N, nu, dt, n, seg = 256, 1.0 / 1600, 2e-3, 6, 2
+    N, nu, dt, n, seg = 256, 1.0 / 1600, 2e-3, 8, 2
     p = _plan(N)
     x = p.forward(torch.tensor(ensemble.perturbed_taylor_green(N, 2024), device="cuda"))
     aws = p.adjoint_workspace(n, seg)
@@ -343,10 +344,10 @@ def test_fused_folded_sweep_at_128_matches_torch_autograd():
     """N = 128 runs the fused and folded reverse sweep: the register forward x
     adds the kx terms of i k x p2 and does the next stage VJP's inverse x of
     its cotangent, the five-field forward y folds the ky terms, the z pass the
-    kz terms.  Three steps in segments of 1 (the first one recomputed on the
+    kz terms.  Four steps in segments of 1 (the first one recomputed on the
     second stream) against the torch autograd restatement, the gradient to
     1e-11 and dJ/dnu to 1e-10 relative."""
-    N, nu, dt, n, seg = 128, 1.0 / 400, 5e-3, 3, 1
+    N, nu, dt, n, seg = 128, 1.0 / 400, 5e-3, 4, 1
     x = _state(N, 12, amp=0.05)
     c = torch.tensor(x, requires_grad=True)
     nut = torch.tensor(nu, dtype=torch.float64, requires_grad=True)
