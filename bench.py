@@ -211,13 +211,17 @@ def run_mhd(N, steps, dt, nu, eta, dtype, lorentz_sign=1.0):
             "cross_helicity": d["cross_helicity"], "div_u_max": d["div_u_max"], "div_b_max": d["div_b_max"]}
 
 
-def run_ensemble(ninst, steps, dt, seed=0):
+def run_ensemble(ninst, steps, dt, seed=0, sizes=None):
     """Seeded 64-instance FP32 ensemble (sizes {32,48,64,96}, nu log-uniform
     in [1/1600, 1/100], TG + |k|<=4 perturbation of rms 0.05): wall time of
-    `steps` RK4 steps, one captured HIP graph replay per step."""
+    `steps` RK4 steps, one captured HIP graph replay per step.  `sizes`
+    keeps only the instances of those grid sizes (for per-size profiles)."""
     import torch
     from jax_tg_project_amd import ensemble, roofline
     cfg = ensemble.ensemble_config(ninst, seed=seed)
+    if sizes:
+        cfg = [c for c in cfg if c[0] in sizes]
+        ninst = len(cfg)
     ens = ensemble.build_ensemble(cfg, dtype=torch.float32)
     layout = ens.tune_launch(dt)  # times both launch layouts, keeps the faster
     ens.step(1, dt, graph=True)  # capture + warm-up
@@ -300,6 +304,8 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--only", choices=("fp32", "ensemble", "mhd", "adjoint"),
                     help="run one leg alone (for profiling) and print its JSON object")
+    ap.add_argument("--ensemble-sizes", type=int, nargs="+", default=None,
+                    help="with --only ensemble: keep only the instances of these grid sizes")
     ap.add_argument("--leg-steps", type=int, default=None,
                     help="steps of the --only leg (default: the leg's workload)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -348,7 +354,7 @@ def main():
                    "passes_fp32": {k: {"ms_avg": v[0] / max(1, v[1]), "launches": v[1]}
                                    for k, v in r["passes"].items()}}
         elif args.only == "ensemble":
-            out = run_ensemble(64, ls or 200, 5e-3)
+            out = run_ensemble(64, ls or 200, 5e-3, sizes=args.ensemble_sizes)
         elif args.only == "mhd":
             out = {"fp64": run_mhd(128, ls or 300, 2e-3, 1.0 / 200, 1.0 / 200, torch.float64),
                    "fp32": run_mhd(128, ls or 300, 2e-3, 1.0 / 200, 1.0 / 200, torch.float32)}
