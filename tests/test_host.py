@@ -229,3 +229,27 @@ def test_native_cli_argument_errors():
                  ["16", "0.01", "100", "1", "0.05", "--precision", "16"]):
         r = subprocess.run([exe] + args, capture_output=True, text=True)
         assert r.returncode == 1 and "Usage:" in r.stderr, (args, r.returncode, r.stderr)
+
+
+def test_register_z_lds_table_matches_the_bank_model():
+    """The FP32 entries of the ZwLds table (tg_passes.hpp) are the layouts the
+    LDS bank model (csrc/zw_lds_model.py) gives the fewest cycles, and each
+    costs fewer modeled cycles than the default layout; the FP64 entry is a
+    measured choice and only has to be a valid layout."""
+    csrc = os.path.join(ROOT, "jax_tg_project_amd", "csrc")
+    sys.path.insert(0, csrc)
+    try:
+        import zw_lds_model as m
+    finally:
+        sys.path.remove(csrc)
+    src = open(os.path.join(csrc, "tg_passes.hpp")).read()
+    rows = re.findall(r"^TG_ZW_LDS\((float|double), (\d+), (\d+), (\d+), (\d+)\)", src, re.M)
+    assert rows, "no ZwLds entries found"
+    for prec, n, e, ps, lpl in rows:
+        n, e, ps, lpl = int(n), int(e), int(ps), int(lpl)
+        assert lpl >= n + (n >> ps)
+        if prec == "double":
+            continue
+        cost, best_lpl, best_ps = m.best_layout(n, e, 4)
+        assert (best_ps, best_lpl) == (ps, lpl), (n, e, ps, lpl, best_ps, best_lpl)
+        assert cost < m.cost(n, e, 4, *m.default_layout(n, e, 4)), (n, e)
