@@ -221,7 +221,7 @@ int tg_adjoint_workspace(tg_plan* plan, int nsteps, int seg, long long* bytes);
 /* Forward sweep: J = E(T) (objective 0) or Z(T) (1) after nsteps RK4 steps of
  * size dt from x0_hat with viscosity nu, keeping a checkpoint every `seg`
  * steps (seg = ceil(sqrt(nsteps)) is the sqrt-n schedule) in adj_ws, and
- * the stage states of the last three segments (so the reverse sweep never
+ * the stage states of the last four segments (so the reverse sweep never
  * recomputes them; seg = nsteps keeps every stage and recomputes nothing).
  * Synchronises `stream` (J is returned to the host). */
 int tg_adjoint_forward(tg_plan* plan, const void* x0_hat, int nsteps, double dt, double nu, int objective, int seg,

@@ -461,18 +461,17 @@ struct Ops {
   // 4 seg stage states | RHS workspace.
   // On grids with the register adjoint z pass, each stage buffer has a
   // matching set of 4 seg slots for the stages' inverse-y output (u, w).
-  // With more than two segments a third stage buffer (4 seg stage states,
-  // then its slots) holds the third-last segment, which the forward sweep
-  // keeps as well.
+  // The forward sweep keeps the last kKeptSegments segments: each one past
+  // the second has a stage buffer of its own (4 seg stage states, then its
+  // slots) after those, when there are that many segments.
+  static constexpr int kKeptSegments = kAdjKeptSegments;
   struct AdjLayout {
     cpx<R>* cur; cpx<R>* lamp; cpx<R>* lacc; cpx<R>* abuf; cpx<R>* ck; cpx<R>* st; double* nupart;
     cpx<R>* cur2; cpx<R>* st2; void* ws2;
     cpx<R>* slots; cpx<R>* slots2;
-    cpx<R>* st3; cpx<R>* slots3;
+    cpx<R>* stk[kKeptSegments]; cpx<R>* slk[kKeptSegments];  // stage buffer k: states, slots
     size_t vstride, sstride; int nck, nblk;
   };
-  // Segments the forward sweep keeps (their stage states): the last three.
-  static constexpr int kKeptSegments = 3;
   // one padded 3-component vector
   static size_t vec_bytes(const tg_plan* p) { return align256(size_t(3) * p->K * p->K * kzp<R, N>() * sizeof(cpx<R>)); }
   static size_t slot_bytes(const tg_plan* p) { return align256(size_t(6) * N * N * kzp<R, N>() * sizeof(cpx<R>)); }
@@ -497,9 +496,17 @@ struct Ops {
     const size_t sb = slot_bytes(p);
     L.slots = adj_reg_z<N>() ? reinterpret_cast<cpx<R>*>(third) : nullptr;
     L.slots2 = (adj_reg_z<N>() && L.nck > 1) ? reinterpret_cast<cpx<R>*>(third + 4 * size_t(seg) * sb) : nullptr;
-    char* fourth = third + (adj_reg_z<N>() ? size_t(L.nck > 1 ? 2 : 1) * 4 * size_t(seg) * sb : 0);
-    L.st3 = L.nck > 2 ? reinterpret_cast<cpx<R>*>(fourth) : nullptr;
-    L.slots3 = (adj_reg_z<N>() && L.nck > 2) ? reinterpret_cast<cpx<R>*>(fourth + 4 * size_t(seg) * vb) : nullptr;
+    char* extra = third + (adj_reg_z<N>() ? size_t(L.nck > 1 ? 2 : 1) * 4 * size_t(seg) * sb : 0);
+    L.stk[0] = L.st;
+    L.slk[0] = L.slots;
+    L.stk[1] = L.st2;
+    L.slk[1] = L.slots2;
+    for (int k = 2; k < kKeptSegments; ++k) {
+      const bool has = L.nck > k;
+      L.stk[k] = has ? reinterpret_cast<cpx<R>*>(extra) : nullptr;
+      L.slk[k] = (has && adj_reg_z<N>()) ? reinterpret_cast<cpx<R>*>(extra + 4 * size_t(seg) * vb) : nullptr;
+      if (has) extra += 4 * size_t(seg) * (vb + (adj_reg_z<N>() ? sb : 0));
+    }
     L.sstride = sb / sizeof(cpx<R>);
     L.vstride = vb / sizeof(cpx<R>);
     L.nblk = Launch<R, N>::adj_x_blocks();
@@ -545,20 +552,21 @@ struct Ops {
     }
     return TG_OK;
   }
-  // Stage buffer (states, slots) of segment c in the reverse sweep: the last
-  // segment in the first buffer, the second-last in the second, and from the
-  // third-last down the third and the first in turn (segment c - 1 is
-  // recomputed into the buffer segment c + 1 used, or into the first for
-  // the fourth-last segment, while segment c is reversed).
+  // Stage buffer (states, slots) of segment c in the reverse sweep: the
+  // i-th last segment (i = nck - 1 - c) in buffer i while i < kKeptSegments,
+  // then the first and the last kept buffer in turn (segment c - 1 is
+  // recomputed into the buffer whose segment was reversed before segment c,
+  // while segment c is reversed).
   static std::pair<cpx<R>*, cpx<R>*> stage_buffer(const AdjLayout& L, int c) {
-    if (c == L.nck - 1) return {L.st, L.slots};
-    if (c == L.nck - 2) return {L.st2, L.slots2};
-    return (L.nck - 3 - c) % 2 == 0 ? std::make_pair(L.st3, L.slots3) : std::make_pair(L.st, L.slots);
+    const int i = L.nck - 1 - c;
+    if (i < kKeptSegments) return {L.stk[i], L.slk[i]};
+    const int k = (i - (kKeptSegments - 1)) % 2 == 0 ? kKeptSegments - 1 : 0;
+    return {L.stk[k], L.slk[k]};
   }
   // Forward sweep: nsteps RK4 steps from x0 with a checkpoint every `seg`
   // steps, J = E(T) (objective 0) or Z(T) (1), and the terminal cotangent.
-  // The stage states (and kept inverse-y outputs) of the last three segments
-  // go to their stage buffers (stage_buffer).
+  // The stage states (and kept inverse-y outputs) of the last kKeptSegments
+  // segments go to their stage buffers (stage_buffer).
   static int adj_forward(const tg_plan* p, const cpx<R>* x0, int nsteps, double dt, double nu, int objective,
                          int seg, double* J, void* ws, void* aws, hipStream_t s) {
     AdjLayout L = adj_layout(p, nsteps, seg, aws);
@@ -602,8 +610,8 @@ struct Ops {
     const R h = R(dt);
     const R cl[4] = {h / R(6), h / R(3), h / R(3), h / R(6)};  // weight of lamp in a_s (s = 1..4)
     const R cm[4] = {R(0), h / R(2), h / R(2), h};              // weight of mu_{s+1} in a_s
-    // The stages of the last three segments are kept by the forward sweep
-    // (stage_buffer).  Segment c - 1 is recomputed on the private stream
+    // The stages of the last kKeptSegments segments are kept by the forward
+    // sweep (stage_buffer).  Segment c - 1 is recomputed on the private stream
     // into its buffer while segment c is reversed on `s`; events order the
     // buffer reuse both ways.
     hipStream_t s2 = nullptr;
@@ -633,7 +641,7 @@ struct Ops {
       const auto bc = stage_buffer(L, c);
       cpx<R>* stc = bc.first;
       cpx<R>* slc = bc.second;
-      // segment c recomputed (the last three were kept by the forward sweep)
+      // segment c recomputed (the last ones were kept by the forward sweep)
       if (c < L.nck - kKeptSegments) TG_HIP(hipStreamWaitEvent(s, ev_rec(c), 0));
       // recompute segment c - 1 into its buffer (free: its last user was
       // reversed earlier); its steps are enqueued one per reversed step of

@@ -329,8 +329,8 @@ int tg_adjoint_workspace(tg_plan* p, int nsteps, int seg, long long* bytes) {
   const size_t sb = align256(6 * size_t(p->n) * p->n * kzpad * p->cb);
   if (adj_reg_z_rt(p->n))  // kept inverse-y output (u, w) per stage, per stage buffer
     b += (nck > 1 ? 2 : 1) * 4 * size_t(seg) * sb;
-  if (nck > 2)  // third stage buffer (the third-last segment is kept too)
-    b += 4 * size_t(seg) * (vb + (adj_reg_z_rt(p->n) ? sb : 0));
+  for (int k = 2; k < kAdjKeptSegments; ++k)  // a stage buffer per further kept segment
+    if (nck > size_t(k)) b += 4 * size_t(seg) * (vb + (adj_reg_z_rt(p->n) ? sb : 0));
   *bytes = (long long)b;
   return TG_OK;
 }
